@@ -1,0 +1,302 @@
+// C ABI of the sort: descriptor validation, plan and workspace layout,
+// dispatch to the per-key-width launchers.
+//
+// Reference counterparts:
+//   Sort_Workspace::build/bind        include/rsort/detail/radix_kernel.cuh:35-109
+//   onesweep_byte_sort_impl (query)   include/rsort/detail/radix_kernel.cuh:486-515
+//   lookback_policy_enforcer          include/rsort/detail/radix_kernel.cuh:525-559
+//   long double -> 128-bit word       include/rsort/detail/radix_kernel.cuh:351-356
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+
+#include "../../include/rsort_c.h"
+#include "rsort_kernels.hpp"
+#include "rsort_launch.hpp"
+
+namespace rsa {
+
+namespace {
+
+thread_local char g_last_error[512] = "";
+
+rsort_status fail(rsort_status s, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+rsort_status fail(rsort_status s, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+    return s;
+}
+
+rsort_status hip_fail(hipError_t e, const char* what) {
+    return fail(RSORT_ERROR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+size_t reserve(size_t* off, size_t bytes, size_t align = 256) {
+    const size_t at = (*off + align - 1) / align * align;
+    *off = at + bytes;
+    return at;
+}
+
+bool is_pow2_width(uint32_t b) { return b == 1 || b == 2 || b == 4 || b == 8 || b == 16; }
+
+struct Plan {
+    uint32_t kind, kb, vb_user, vb_core;
+    bool desc, allow_skip, index_mode, wide;
+    uint32_t passes;
+    TileCfg cfg;
+    uint64_t tile, num_tiles;
+    size_t off_tmp_keys, off_tmp_vals, off_idx, off_vals_copy, off_ctrl, off_counts, off_gp, off_lb;
+    size_t lb_bytes, total;
+};
+
+rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
+    if (!d) return fail(RSORT_ERROR_INVALID_ARGUMENT, "descriptor is NULL");
+    if (d->reserved != 0) return fail(RSORT_ERROR_INVALID_ARGUMENT, "descriptor reserved field must be 0");
+    if (d->key_kind > RSORT_KEY_FLOAT) return fail(RSORT_ERROR_INVALID_ARGUMENT, "unknown key kind %u", d->key_kind);
+    if (!is_pow2_width(d->key_bytes))
+        return fail(RSORT_ERROR_UNSUPPORTED, "key width %u bytes not supported (1,2,4,8,16)", d->key_bytes);
+    if (d->key_kind == RSORT_KEY_FLOAT && d->key_bytes == 1)
+        return fail(RSORT_ERROR_UNSUPPORTED, "no 1-byte floating key type");
+    if (d->descending > 1) return fail(RSORT_ERROR_INVALID_ARGUMENT, "descending must be 0 or 1");
+    if (d->flags & ~RSORT_FLAG_NO_PASS_SKIP) return fail(RSORT_ERROR_INVALID_ARGUMENT, "unknown flags 0x%x", d->flags);
+
+    memset(p, 0, sizeof(*p));
+    p->kind = d->key_kind;
+    p->kb = d->key_bytes;
+    p->vb_user = d->value_bytes;
+    p->desc = d->descending != 0;
+    p->allow_skip = (d->flags & RSORT_FLAG_NO_PASS_SKIP) == 0;
+    p->index_mode = d->value_bytes != 0 && !is_pow2_width(d->value_bytes);
+    p->vb_core = p->index_mode ? 4u : d->value_bytes;
+    if (p->index_mode && n > 0xFFFFFFFFull)
+        return fail(RSORT_ERROR_UNSUPPORTED, "values of %u bytes need n < 2^32", d->value_bytes);
+    p->wide = n > kNarrowMaxN;
+    p->passes = p->kb;
+    p->cfg = tile_cfg((int)p->kb, (int)p->vb_core);
+    p->tile = (uint64_t)p->cfg.threads * p->cfg.items;
+    p->num_tiles = n ? (n + p->tile - 1) / p->tile : 0;
+    if (p->num_tiles > 0x7FFFFFFFull) return fail(RSORT_ERROR_UNSUPPORTED, "n too large");
+
+    const size_t off_sz = p->wide ? 8 : 4;
+    size_t off = 0;
+    p->off_tmp_keys = reserve(&off, (size_t)n * p->kb);
+    p->off_tmp_vals = p->vb_core ? reserve(&off, (size_t)n * p->vb_core) : 0;
+    if (p->index_mode) {
+        p->off_idx = reserve(&off, (size_t)n * 4u);
+        p->off_vals_copy = reserve(&off, (size_t)n * p->vb_user);
+    }
+    p->off_ctrl = reserve(&off, sizeof(Ctrl));
+    p->off_counts = reserve(&off, (size_t)p->passes * kRadix * off_sz, 4);
+    p->off_gp = reserve(&off, (size_t)p->passes * kRadix * off_sz, 4);
+    p->lb_bytes = (size_t)p->num_tiles * kRadix * off_sz;
+    p->off_lb = reserve(&off, p->lb_bytes);
+    p->total = off + 256;  // slack so any 16-byte-aligned workspace pointer can be rounded up
+    return RSORT_SUCCESS;
+}
+
+int device_cus(int dev) {
+    static std::mutex mu;
+    static int cache[64] = {0};
+    if (dev < 0 || dev >= 64) return 256;
+    std::lock_guard<std::mutex> lk(mu);
+    if (cache[dev] == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+        cache[dev] = v;
+    }
+    return cache[dev];
+}
+
+hipError_t dispatch_core(const SortJob& j, hipStream_t s) {
+    switch (j.kb) {
+        case 1: return launch_core_kb1(j, s);
+        case 2: return launch_core_kb2(j, s);
+        case 4: return launch_core_kb4(j, s);
+        case 8: return launch_core_kb8(j, s);
+        case 16: return launch_core_kb16(j, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* workspace, size_t* temp_bytes, uint64_t n,
+                      void* stream, hipEvent_t* events, int max_events, int* n_events) {
+    Plan p;
+    rsort_status st = make_plan(d, n, &p);
+    if (st != RSORT_SUCCESS) return st;
+    if (!temp_bytes) return fail(RSORT_ERROR_INVALID_ARGUMENT, "temp_bytes is NULL");
+    if (!workspace) {
+        *temp_bytes = p.total;
+        return RSORT_SUCCESS;
+    }
+    if (*temp_bytes < p.total)
+        return fail(RSORT_ERROR_WORKSPACE_TOO_SMALL, "workspace of %zu bytes, %zu required", *temp_bytes, p.total);
+    if (n <= 1) return RSORT_SUCCESS;
+    if (!keys) return fail(RSORT_ERROR_INVALID_ARGUMENT, "keys is NULL");
+    if (p.vb_user && !values) return fail(RSORT_ERROR_INVALID_ARGUMENT, "values is NULL but value_bytes = %u", p.vb_user);
+    if ((uintptr_t)keys % p.kb) return fail(RSORT_ERROR_INVALID_ARGUMENT, "keys not aligned to %u bytes", p.kb);
+    if (p.vb_user && is_pow2_width(p.vb_user) && (uintptr_t)values % p.vb_user)
+        return fail(RSORT_ERROR_INVALID_ARGUMENT, "values not aligned to %u bytes", p.vb_user);
+    if ((uintptr_t)workspace % 16) return fail(RSORT_ERROR_INVALID_ARGUMENT, "workspace not 16-byte aligned");
+
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    const int cus = device_cus(dev);
+    hipStream_t s = (hipStream_t)stream;
+
+    uint8_t* base = (uint8_t*)(((uintptr_t)workspace + 255u) & ~(uintptr_t)255u);
+    SortJob j{};
+    j.kb = p.kb;
+    j.vb = p.vb_core;
+    j.wide = p.wide;
+    j.allow_skip = p.allow_skip;
+    j.tw = make_tw_masks(p.kind, p.kb, p.desc);
+    j.keys = keys;
+    j.tmp_keys = base + p.off_tmp_keys;
+    j.tmp_vals = p.vb_core ? base + p.off_tmp_vals : nullptr;
+    j.vals = p.index_mode ? (void*)(base + p.off_idx) : values;
+    j.ctrl = (Ctrl*)(base + p.off_ctrl);
+    j.counts = base + p.off_counts;
+    j.gp = base + p.off_gp;
+    j.lookback = base + p.off_lb;
+    j.lb_bytes = p.lb_bytes;
+    j.n = n;
+    j.num_tiles = p.num_tiles;
+    j.passes = p.passes;
+    j.num_cus = cus;
+    j.events = events;
+    j.max_events = max_events;
+    j.n_events = n_events;
+
+    if (p.index_mode) {
+        e = launch_iota((uint32_t*)j.vals, n, cus, s);
+        if (e != hipSuccess) return hip_fail(e, "index fill");
+    }
+    e = dispatch_core(j, s);
+    if (e != hipSuccess) return hip_fail(e, "sort launch");
+    if (p.index_mode) {
+        void* copy = base + p.off_vals_copy;
+        e = launch_gather(copy, values, (const uint32_t*)j.vals, n, p.vb_user, cus, s);
+        if (e == hipSuccess) e = launch_copy(values, copy, n * p.vb_user, cus, s);
+        if (e != hipSuccess) return hip_fail(e, "value gather");
+    }
+    return RSORT_SUCCESS;
+}
+
+}  // namespace
+
+hipError_t launch_iota(uint32_t* p, uint64_t n, int num_cus, hipStream_t s) {
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 1024, num_cus, 8)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(void* dst, const void* src, const uint32_t* idx, uint64_t n, uint32_t rec_bytes,
+                         int num_cus, hipStream_t s) {
+    const uintptr_t al = (uintptr_t)dst | (uintptr_t)src | rec_bytes;
+    const dim3 grid(grid_for(n * rec_bytes, 4096, num_cus, 8)), block(256);
+    if ((al & 3u) == 0) {
+        hipLaunchKernelGGL(k_gather<uint32_t>, grid, block, 0, s, (uint32_t*)dst, (const uint32_t*)src, idx, n,
+                           rec_bytes / 4u);
+    } else if ((al & 1u) == 0) {
+        hipLaunchKernelGGL(k_gather<uint16_t>, grid, block, 0, s, (uint16_t*)dst, (const uint16_t*)src, idx, n,
+                           rec_bytes / 2u);
+    } else {
+        hipLaunchKernelGGL(k_gather<uint8_t>, grid, block, 0, s, (uint8_t*)dst, (const uint8_t*)src, idx, n,
+                           rec_bytes);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, int num_cus, hipStream_t s) {
+    const uintptr_t al = (uintptr_t)dst | (uintptr_t)src | (uintptr_t)bytes;
+    const dim3 block(256);
+    if ((al & 15u) == 0) {
+        hipLaunchKernelGGL(k_copy<uint4>, dim3(grid_for(bytes / 16, 1024, num_cus, 8)), block, 0, s, (uint4*)dst,
+                           (const uint4*)src, bytes / 16);
+    } else if ((al & 3u) == 0) {
+        hipLaunchKernelGGL(k_copy<uint32_t>, dim3(grid_for(bytes / 4, 1024, num_cus, 8)), block, 0, s,
+                           (uint32_t*)dst, (const uint32_t*)src, bytes / 4);
+    } else {
+        hipLaunchKernelGGL(k_copy<uint8_t>, dim3(grid_for(bytes, 1024, num_cus, 8)), block, 0, s, (uint8_t*)dst,
+                           (const uint8_t*)src, bytes);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace rsa
+
+extern "C" {
+
+rsort_status rsort_sort(const rsort_desc* desc, void* keys, void* values, void* workspace, size_t* temp_bytes,
+                        uint64_t n, void* stream) {
+    return rsa::run_sort(desc, keys, values, workspace, temp_bytes, n, stream, nullptr, 0, nullptr);
+}
+
+rsort_status rsort_sort_profiled(const rsort_desc* desc, void* keys, void* values, void* workspace,
+                                 size_t* temp_bytes, uint64_t n, void* stream, float* stage_ms, int max_stages,
+                                 int* n_stages) {
+    if (n_stages) *n_stages = 0;
+    if (!workspace) return rsa::run_sort(desc, keys, values, workspace, temp_bytes, n, stream, nullptr, 0, nullptr);
+    constexpr int kMaxEv = 40;
+    hipEvent_t ev[kMaxEv];
+    int made = 0;
+    for (; made < kMaxEv; ++made) {
+        if (hipEventCreate(&ev[made]) != hipSuccess) break;
+    }
+    int used = 0;
+    rsort_status st = rsa::run_sort(desc, keys, values, workspace, temp_bytes, n, stream, ev, made, &used);
+    if (st == RSORT_SUCCESS && used > 0) {
+        hipError_t e = hipEventSynchronize(ev[used - 1]);
+        if (e != hipSuccess) st = rsa::hip_fail(e, "hipEventSynchronize");
+        int k = 0;
+        for (int i = 1; st == RSORT_SUCCESS && i < used && k < max_stages; ++i, ++k) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[i - 1], ev[i]);
+            if (stage_ms) stage_ms[k] = ms;
+        }
+        if (n_stages) *n_stages = k;
+    }
+    for (int i = 0; i < made; ++i) (void)hipEventDestroy(ev[i]);
+    return st;
+}
+
+rsort_status rsort_plan(const rsort_desc* desc, uint64_t n, rsort_plan_info* out) {
+    if (!out) return rsa::fail(RSORT_ERROR_INVALID_ARGUMENT, "plan output is NULL");
+    rsa::Plan p;
+    rsort_status st = rsa::make_plan(desc, n, &p);
+    if (st != RSORT_SUCCESS) return st;
+    out->passes = p.passes;
+    out->threads = p.cfg.threads;
+    out->items_per_thread = p.cfg.items;
+    out->tile = (uint32_t)p.tile;
+    out->num_tiles = p.num_tiles;
+    out->wide = p.wide;
+    out->index_mode = p.index_mode;
+    out->workspace_bytes = p.total;
+    out->lookback_bytes = p.lb_bytes;
+    return RSORT_SUCCESS;
+}
+
+const char* rsort_status_string(rsort_status s) {
+    switch (s) {
+        case RSORT_SUCCESS: return "success";
+        case RSORT_ERROR_INVALID_ARGUMENT: return "invalid argument";
+        case RSORT_ERROR_UNSUPPORTED: return "unsupported";
+        case RSORT_ERROR_WORKSPACE_TOO_SMALL: return "workspace too small";
+        case RSORT_ERROR_DEVICE: return "device error";
+        case RSORT_ERROR_BUSY: return "busy";
+    }
+    return "unknown status";
+}
+
+const char* rsort_last_error(void) { return rsa::g_last_error; }
+
+int rsort_abi_version(void) { return RSORT_ABI_VERSION; }
+
+}  // extern "C"

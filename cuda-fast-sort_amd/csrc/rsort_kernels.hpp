@@ -1,0 +1,729 @@
+// gfx950 kernels of the onesweep LSD radix sort (8-bit digits).
+//
+//   k_histogram  - one HBM read counts every digit of every pass into
+//                  wave-private LDS counters, folds them into global counts and
+//                  (last workgroup) turns them into exclusive digit offsets and
+//                  the uniform-pass skip mask.
+//                  Reference: include/rsort/detail/histogram.cuh:88-211.
+//   k_onesweep   - one stable counting-sort pass: load tile, rank in LDS,
+//                  decoupled lookback on a ticketed tile order, LDS-staged
+//                  coalesced scatter of keys then values.
+//                  Reference: include/rsort/detail/radix_kernel.cuh:118-294,
+//                  ranker.cuh:62-170, lookback.cuh:252-303, scatter.cuh:159-202.
+//   k_fill_zero, k_copy_back, k_iota, k_gather - small helpers of the op
+//                  sequence (radix_kernel.cuh:377-386, 468-473).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "rsort_protocol.hpp"
+#include "rsort_tuning.hpp"
+
+namespace rsa {
+
+// ---------------------------------------------------------------------------
+// Key and value storage traits
+// ---------------------------------------------------------------------------
+struct alignas(16) U128 {
+    uint64_t lo, hi;
+};
+
+template <int KB>
+struct KeyTraits;
+
+template <>
+struct KeyTraits<1> {
+    using Mem = uint8_t;
+    using Reg = uint32_t;
+};
+template <>
+struct KeyTraits<2> {
+    using Mem = uint16_t;
+    using Reg = uint32_t;
+};
+template <>
+struct KeyTraits<4> {
+    using Mem = uint32_t;
+    using Reg = uint32_t;
+};
+template <>
+struct KeyTraits<8> {
+    using Mem = uint64_t;
+    using Reg = uint64_t;
+};
+template <>
+struct KeyTraits<16> {
+    using Mem = U128;
+    using Reg = U128;
+};
+
+template <int VB>
+struct ValTraits;
+template <>
+struct ValTraits<0> {
+    using Mem = uint8_t;  // unused placeholder
+};
+template <>
+struct ValTraits<1> {
+    using Mem = uint8_t;
+};
+template <>
+struct ValTraits<2> {
+    using Mem = uint16_t;
+};
+template <>
+struct ValTraits<4> {
+    using Mem = uint32_t;
+};
+template <>
+struct ValTraits<8> {
+    using Mem = uint64_t;
+};
+template <>
+struct ValTraits<16> {
+    using Mem = U128;
+};
+
+// ---------------------------------------------------------------------------
+// Register-level key operations
+// ---------------------------------------------------------------------------
+template <int KB>
+__device__ __forceinline__ uint32_t key_filler32() {
+    if constexpr (KB >= 4) {
+        return 0xFFFFFFFFu;
+    } else {
+        return (1u << (8 * KB)) - 1u;
+    }
+}
+
+template <int KB>
+__device__ __forceinline__ typename KeyTraits<KB>::Reg key_filler() {
+    if constexpr (KB <= 4) {
+        return key_filler32<KB>();
+    } else if constexpr (KB == 8) {
+        return ~0ull;
+    } else {
+        return U128{~0ull, ~0ull};
+    }
+}
+
+// Twiddle in: order-preserving map to unsigned (radix_traits::twiddle_in,
+// tuning.cuh:132-148), with the descending complement folded in.
+template <int KB>
+__device__ __forceinline__ typename KeyTraits<KB>::Reg tw_in(typename KeyTraits<KB>::Reg x, const TwMasks& t) {
+    if constexpr (KB <= 4) {
+        constexpr int SH = 32 - 8 * KB;
+        const uint32_t neg = (uint32_t)(((int32_t)(x << SH)) >> 31);
+        return x ^ ((neg & (uint32_t)t.fmask) | (uint32_t)t.smask) ^ (uint32_t)t.dmask;
+    } else if constexpr (KB == 8) {
+        const uint64_t neg = (uint64_t)(((int64_t)x) >> 63);
+        return x ^ ((neg & t.fmask) | t.smask) ^ t.dmask;
+    } else {
+        const uint64_t neg = (uint64_t)(((int64_t)x.hi) >> 63);
+        U128 r;
+        r.lo = x.lo ^ (neg & t.fmask) ^ t.dmask;
+        r.hi = x.hi ^ ((neg & t.fmask) | t.smask) ^ t.dmask;
+        return r;
+    }
+}
+
+// Twiddle out: inverse of tw_in (radix_traits::twiddle_out, tuning.cuh:151-165).
+template <int KB>
+__device__ __forceinline__ typename KeyTraits<KB>::Reg tw_out(typename KeyTraits<KB>::Reg x, const TwMasks& t) {
+    if constexpr (KB <= 4) {
+        constexpr int SH = 32 - 8 * KB;
+        const uint32_t u = x ^ (uint32_t)t.dmask;
+        const uint32_t neg = (uint32_t)(((int32_t)(u << SH)) >> 31);
+        return u ^ ((~neg & (uint32_t)t.fmask) | (uint32_t)t.smask);
+    } else if constexpr (KB == 8) {
+        const uint64_t u = x ^ t.dmask;
+        const uint64_t neg = (uint64_t)(((int64_t)u) >> 63);
+        return u ^ ((~neg & t.fmask) | t.smask);
+    } else {
+        U128 u{x.lo ^ t.dmask, x.hi ^ t.dmask};
+        const uint64_t neg = (uint64_t)(((int64_t)u.hi) >> 63);
+        u.lo ^= (~neg & t.fmask);
+        u.hi ^= ((~neg & t.fmask) | t.smask);
+        return u;
+    }
+}
+
+// Digit of a pass: byte (8*pass) of the key; `sh` = 8*(pass&3) selects the
+// byte in a dword, `wsel` = pass>>2 the dword (both wave-uniform).
+__device__ __forceinline__ uint32_t key_digit(uint32_t x, uint32_t sh, uint32_t) {
+    return __builtin_amdgcn_ubfe(x, sh, 8);
+}
+__device__ __forceinline__ uint32_t key_digit(uint64_t x, uint32_t sh, uint32_t wsel) {
+    const uint32_t w = wsel ? (uint32_t)(x >> 32) : (uint32_t)x;
+    return __builtin_amdgcn_ubfe(w, sh, 8);
+}
+__device__ __forceinline__ uint32_t key_digit(const U128& x, uint32_t sh, uint32_t wsel) {
+    const uint64_t h = (wsel & 2u) ? x.hi : x.lo;
+    const uint32_t w = (wsel & 1u) ? (uint32_t)(h >> 32) : (uint32_t)h;
+    return __builtin_amdgcn_ubfe(w, sh, 8);
+}
+
+template <int KB>
+__device__ __forceinline__ typename KeyTraits<KB>::Reg key_to_reg(typename KeyTraits<KB>::Mem m) {
+    return (typename KeyTraits<KB>::Reg)m;
+}
+template <>
+__device__ __forceinline__ U128 key_to_reg<16>(U128 m) {
+    return m;
+}
+template <int KB>
+__device__ __forceinline__ typename KeyTraits<KB>::Mem key_to_mem(typename KeyTraits<KB>::Reg r) {
+    return (typename KeyTraits<KB>::Mem)r;
+}
+template <>
+__device__ __forceinline__ U128 key_to_mem<16>(U128 r) {
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Wave / LDS helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Number of set bits of a 64-lane mask below the calling lane.
+__device__ __forceinline__ uint32_t mask_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T x) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const T y = __shfl_up(x, (unsigned)o);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    return x;
+}
+
+// Compiler-only barrier: keeps the per-item LDS sequence of the ranker in
+// program order (the LDS executes one wave's instructions in order).
+#define RSA_COMPILER_FENCE() __atomic_signal_fence(__ATOMIC_SEQ_CST)
+
+template <typename T>
+__device__ __forceinline__ T lds_ld(T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+template <typename T>
+__device__ __forceinline__ void lds_st(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+// ---------------------------------------------------------------------------
+// Kernel parameter blocks
+// ---------------------------------------------------------------------------
+struct HistParams {
+    const void* keys;      // caller's keys (original representation)
+    void* counts;          // [passes][256] Off, zeroed by k_fill_zero
+    void* gp;              // [passes][256] Off exclusive digit offsets (out)
+    Ctrl* ctrl;
+    uint4* lb_zero;        // lookback array to clear (16-byte units)
+    uint64_t lb_zero_units;
+    uint64_t n;
+    uint64_t head;         // keys before the first 16-byte aligned group
+    uint32_t passes;
+    uint32_t allow_skip;
+    TwMasks tw;
+};
+
+struct PassParams {
+    void* keys_a;          // caller's buffer (pass input when executed-pass index is even)
+    void* keys_b;          // workspace ping-pong buffer
+    void* vals_a;
+    void* vals_b;
+    const void* gp;        // [passes][256] exclusive digit offsets
+    void* lookback;        // [tiles][256] words
+    Ctrl* ctrl;
+    uint64_t n;
+    uint32_t pass;
+    uint32_t passes;
+    TwMasks tw;
+};
+
+// ---------------------------------------------------------------------------
+// Helpers
+// ---------------------------------------------------------------------------
+static __global__ void k_fill_zero(uint32_t* __restrict__ p, uint32_t words) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Odd number of executed passes leaves the result in the workspace buffer:
+// copy it back (radix_kernel.cuh:468-473), decided on the device because
+// uniform passes are skipped on the device.
+template <typename E>
+__global__ void k_copy_back(const Ctrl* __restrict__ ctrl, uint32_t passes, E* __restrict__ dst,
+                            const E* __restrict__ src, uint64_t elems) {
+    const uint32_t pmask = passes >= 32 ? ~0u : ((1u << passes) - 1u);
+    const uint32_t active = ~ctrl->skip_mask & pmask;
+    if ((__builtin_popcount(active) & 1u) == 0u) return;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < elems; i += stride) dst[i] = src[i];
+}
+
+template <typename E>
+__global__ void k_copy(E* __restrict__ dst, const E* __restrict__ src, uint64_t elems) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < elems; i += stride) dst[i] = src[i];
+}
+
+static __global__ void k_iota(uint32_t* __restrict__ p, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = (uint32_t)i;
+}
+
+// dst record i = src record idx[i]; records of `rec` elements of type E.
+template <typename E>
+__global__ void k_gather(E* __restrict__ dst, const E* __restrict__ src, const uint32_t* __restrict__ idx,
+                         uint64_t n, uint32_t rec) {
+    const uint64_t total = n * rec;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+        const uint64_t i = t / rec;
+        const uint32_t c = (uint32_t)(t - i * rec);
+        dst[t] = src[(uint64_t)idx[i] * rec + c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Global histogram of all passes (one read).
+// ---------------------------------------------------------------------------
+template <int KB>
+struct HistVec {
+    static constexpr int V = 16 / KB;  // keys per lane per 16-byte load
+    using Reg = typename KeyTraits<KB>::Reg;
+    __device__ __forceinline__ static void unpack(const uint4& q, Reg (&k)[V]) {
+        if constexpr (KB == 1) {
+            const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int j = 0; j < 16; ++j) k[j] = (w[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+        } else if constexpr (KB == 2) {
+            const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) k[j] = (w[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+        } else if constexpr (KB == 4) {
+            k[0] = q.x;
+            k[1] = q.y;
+            k[2] = q.z;
+            k[3] = q.w;
+        } else if constexpr (KB == 8) {
+            k[0] = (uint64_t)q.x | ((uint64_t)q.y << 32);
+            k[1] = (uint64_t)q.z | ((uint64_t)q.w << 32);
+        } else {
+            k[0].lo = (uint64_t)q.x | ((uint64_t)q.y << 32);
+            k[0].hi = (uint64_t)q.z | ((uint64_t)q.w << 32);
+        }
+    }
+};
+
+// byte p (compile-time) of a register key
+template <int P>
+__device__ __forceinline__ uint32_t reg_byte(uint32_t x) {
+    return __builtin_amdgcn_ubfe(x, 8 * (P & 3), 8);
+}
+template <int P>
+__device__ __forceinline__ uint32_t reg_byte(uint64_t x) {
+    return __builtin_amdgcn_ubfe((uint32_t)(x >> (32 * (P >> 2))), 8 * (P & 3), 8);
+}
+template <int P>
+__device__ __forceinline__ uint32_t reg_byte(const U128& x) {
+    const uint64_t h = (P >= 8) ? x.hi : x.lo;
+    return __builtin_amdgcn_ubfe((uint32_t)(h >> (32 * ((P >> 2) & 1))), 8 * (P & 3), 8);
+}
+
+__device__ __forceinline__ uint32_t rfl(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t rfl(uint64_t x) {
+    return (uint64_t)rfl((uint32_t)x) | ((uint64_t)rfl((uint32_t)(x >> 32)) << 32);
+}
+__device__ __forceinline__ U128 rfl(const U128& x) { return U128{rfl(x.lo), rfl(x.hi)}; }
+__device__ __forceinline__ uint32_t reg_or(uint32_t a, uint32_t b) { return a | b; }
+__device__ __forceinline__ uint64_t reg_or(uint64_t a, uint64_t b) { return a | b; }
+__device__ __forceinline__ U128 reg_or(const U128& a, const U128& b) { return U128{a.lo | b.lo, a.hi | b.hi}; }
+__device__ __forceinline__ uint32_t reg_and(uint32_t a, uint32_t b) { return a & b; }
+__device__ __forceinline__ uint64_t reg_and(uint64_t a, uint64_t b) { return a & b; }
+__device__ __forceinline__ U128 reg_and(const U128& a, const U128& b) { return U128{a.lo & b.lo, a.hi & b.hi}; }
+__device__ __forceinline__ uint32_t reg_xor(uint32_t a, uint32_t b) { return a ^ b; }
+__device__ __forceinline__ uint64_t reg_xor(uint64_t a, uint64_t b) { return a ^ b; }
+__device__ __forceinline__ U128 reg_xor(const U128& a, const U128& b) { return U128{a.lo ^ b.lo, a.hi ^ b.hi}; }
+
+// Count pass P of V keys held by each lane.  If the whole wave's group has one
+// digit in this pass, a single lane adds the group size (avoids 64-way LDS
+// atomic serialisation on uniform bytes).
+template <int KB, int P, int V>
+__device__ __forceinline__ void hist_count_pass(uint32_t* H, const typename KeyTraits<KB>::Reg (&k)[V],
+                                                const typename KeyTraits<KB>::Reg& v,
+                                                const typename KeyTraits<KB>::Reg& o0) {
+    if constexpr (P < KB) {
+        if (!__any(reg_byte<P>(v) != 0u)) {
+            if (lane_id() == 0) atomicAdd(H + P * 256 + reg_byte<P>(o0), (uint32_t)(64 * V));
+        } else {
+#pragma unroll
+            for (int j = 0; j < V; ++j) atomicAdd(H + P * 256 + reg_byte<P>(k[j]), 1u);
+        }
+        hist_count_pass<KB, P + 1, V>(H, k, v, o0);
+    }
+}
+
+template <int KB, int P>
+__device__ __forceinline__ void hist_count_one(uint32_t* H, const typename KeyTraits<KB>::Reg& k) {
+    if constexpr (P < KB) {
+        atomicAdd(H + P * 256 + reg_byte<P>(k), 1u);
+        hist_count_one<KB, P + 1>(H, k);
+    }
+}
+
+template <int KB, bool WIDE>
+__global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
+    using Reg = typename KeyTraits<KB>::Reg;
+    using Mem = typename KeyTraits<KB>::Mem;
+    using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
+    constexpr int P = KB;
+    constexpr int WAVES = kHistThreads / 64;
+    constexpr int COPIES = hist_copies(KB);
+    constexpr int WPC = WAVES / COPIES;
+    constexpr int V = HistVec<KB>::V;
+    constexpr uint32_t G = 64u * V;  // keys per wave group
+    constexpr int UNROLL = 4;
+
+    __shared__ uint32_t s_hist[COPIES * P * 256];
+    __shared__ Off s_wsum[4];
+    __shared__ uint32_t s_last;
+    __shared__ uint32_t s_skip;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t w = tid >> 6;
+    const uint32_t lane = tid & 63u;
+
+    // clear this workgroup's share of the lookback array (used by the passes)
+    {
+        const uint64_t per = (a.lb_zero_units + gridDim.x - 1) / gridDim.x;
+        const uint64_t b0 = per * blockIdx.x;
+        const uint64_t b1 = (b0 + per < a.lb_zero_units) ? b0 + per : a.lb_zero_units;
+        for (uint64_t i = b0 + tid; i < b1; i += kHistThreads) a.lb_zero[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    for (uint32_t i = tid; i < (uint32_t)(COPIES * P * 256); i += kHistThreads) s_hist[i] = 0u;
+    if (tid == 0) s_skip = 0u;
+    __syncthreads();
+
+    uint32_t* H = s_hist + (w / WPC) * (P * 256);
+    const Mem* keys = (const Mem*)a.keys;
+    const bool twid = a.tw.active != 0u;
+
+    const uint64_t body = (a.n - a.head) / G;  // full wave groups
+    const uint64_t g0 = body * blockIdx.x / gridDim.x;
+    const uint64_t g1 = body * (blockIdx.x + 1) / gridDim.x;
+    const uint4* base = (const uint4*)(keys + a.head);
+
+    uint64_t g = g0 + w;
+    for (; g + (UNROLL - 1) * WAVES < g1; g += UNROLL * WAVES) {
+        uint4 q[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) q[u] = base[(g + (uint64_t)u * WAVES) * 64 + lane];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            Reg k[V];
+            HistVec<KB>::unpack(q[u], k);
+            if (twid) {
+#pragma unroll
+                for (int j = 0; j < V; ++j) k[j] = tw_in<KB>(k[j], a.tw);
+            }
+            Reg o = k[0], an = k[0];
+#pragma unroll
+            for (int j = 1; j < V; ++j) {
+                o = reg_or(o, k[j]);
+                an = reg_and(an, k[j]);
+            }
+            const Reg o0 = rfl(o);
+            const Reg v = reg_or(reg_xor(o, an), reg_xor(o, o0));
+            hist_count_pass<KB, 0, V>(H, k, v, o0);
+        }
+    }
+    for (; g < g1; g += WAVES) {
+        Reg k[V];
+        HistVec<KB>::unpack(base[g * 64 + lane], k);
+        if (twid) {
+#pragma unroll
+            for (int j = 0; j < V; ++j) k[j] = tw_in<KB>(k[j], a.tw);
+        }
+        Reg o = k[0], an = k[0];
+#pragma unroll
+        for (int j = 1; j < V; ++j) {
+            o = reg_or(o, k[j]);
+            an = reg_and(an, k[j]);
+        }
+        const Reg o0 = rfl(o);
+        const Reg v = reg_or(reg_xor(o, an), reg_xor(o, o0));
+        hist_count_pass<KB, 0, V>(H, k, v, o0);
+    }
+    // head (before the aligned body) and tail (after it): last workgroup
+    if (blockIdx.x == gridDim.x - 1) {
+        const uint64_t tail0 = a.head + body * G;
+        const uint64_t extra = a.head + (a.n - tail0);
+        for (uint64_t t = tid; t < extra; t += kHistThreads) {
+            const uint64_t idx = t < a.head ? t : tail0 + (t - a.head);
+            Reg k = key_to_reg<KB>(keys[idx]);
+            if (twid) k = tw_in<KB>(k, a.tw);
+            hist_count_one<KB, 0>(H, k);
+        }
+    }
+    __syncthreads();
+
+    Off* counts = (Off*)a.counts;
+    for (uint32_t i = tid; i < (uint32_t)(P * 256); i += kHistThreads) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int c = 0; c < COPIES; ++c) s += s_hist[c * P * 256 + i];
+        if (s) atomicAdd(counts + i, (Off)s);
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) s_last = (atomicAdd(&a.ctrl->hist_done, 1u) == gridDim.x - 1u) ? 1u : 0u;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+
+    // last workgroup: exclusive digit offsets per pass + uniform-pass mask
+    Off* gp = (Off*)a.gp;
+    for (int p = 0; p < P; ++p) {
+        Off c = 0, incl = 0;
+        if (tid < 256) {
+            c = __hip_atomic_load(counts + p * 256 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((uint64_t)c == a.n) atomicOr(&s_skip, 1u << p);
+            incl = wave_incl_scan<Off>(c);
+            if (lane == 63) s_wsum[w] = incl;
+        }
+        __syncthreads();
+        if (tid < 256) {
+            Off off = 0;
+            for (uint32_t ww = 0; ww < w; ++ww) off += s_wsum[ww];
+            gp[p * 256 + tid] = off + incl - c;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) a.ctrl->skip_mask = a.allow_skip ? s_skip : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Onesweep pass.
+// ---------------------------------------------------------------------------
+template <typename W>
+__device__ __forceinline__ W lookback_prefix(W* col, uint32_t tile, uint32_t epoch) {
+    W acc = 0;
+    W* p = col + (size_t)(tile - 1u) * kRadix;
+    while (true) {
+        const W word = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lb_ready(word, epoch)) {
+            acc += lb_value(word);
+            if (lb_flag(word) == kLbPrefix) break;
+            p -= kRadix;
+        } else {
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return acc;
+}
+
+template <int KB, int VB, bool WIDE, int THREADS, int K>
+__global__ __launch_bounds__(THREADS) void k_onesweep(PassParams a) {
+    using KM = typename KeyTraits<KB>::Mem;
+    using KR = typename KeyTraits<KB>::Reg;
+    using VM = typename ValTraits<VB>::Mem;
+    using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
+    constexpr int WAVES = THREADS / 64;
+    constexpr uint32_t TILE = (uint32_t)THREADS * K;
+    constexpr size_t KSTAGE = (size_t)TILE * sizeof(KM);
+    constexpr size_t VSTAGE = VB ? (size_t)TILE * sizeof(VM) : 0;
+    constexpr size_t MATCH = (size_t)WAVES * 256 * sizeof(uint64_t);
+    constexpr size_t BIG = KSTAGE > VSTAGE ? (KSTAGE > MATCH ? KSTAGE : MATCH) : (VSTAGE > MATCH ? VSTAGE : MATCH);
+    static_assert(THREADS % 64 == 0 && THREADS >= 256, "workgroup must hold 256 digit owners");
+    static_assert(TILE <= 65535u * 64u, "tile too large");
+
+    __shared__ __attribute__((aligned(16))) uint8_t s_big[BIG];
+    __shared__ __attribute__((aligned(16))) uint32_t s_cnt[WAVES * 256];
+    __shared__ Off s_base[256];
+    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_tile;
+
+    const uint32_t passes = a.passes;
+    const uint32_t pmask = passes >= 32 ? ~0u : ((1u << passes) - 1u);
+    const uint32_t active = ~a.ctrl->skip_mask & pmask;
+    if (!((active >> a.pass) & 1u)) return;  // uniform digit: pass is the identity
+    const uint32_t e = __builtin_popcount(active & ((1u << a.pass) - 1u));
+    const uint32_t epoch = e & 1u;
+    const bool first = (e == 0u) && a.tw.active;
+    const bool last = ((active >> (a.pass + 1u)) == 0u) && a.tw.active;
+    const KM* __restrict__ kin = (const KM*)((e & 1u) ? a.keys_b : a.keys_a);
+    KM* __restrict__ kout = (KM*)((e & 1u) ? a.keys_a : a.keys_b);
+    const VM* __restrict__ vin = (const VM*)((e & 1u) ? a.vals_b : a.vals_a);
+    VM* __restrict__ vout = (VM*)((e & 1u) ? a.vals_a : a.vals_b);
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t w = tid >> 6;
+
+    if (tid == 0) s_tile = atomicAdd(&a.ctrl->ticket[a.pass], 1u);
+    {
+        // each wave clears its own match table and digit counters
+        uint4* m = (uint4*)(s_big + (size_t)w * 256 * sizeof(uint64_t));
+        m[lane] = make_uint4(0u, 0u, 0u, 0u);
+        m[lane + 64] = make_uint4(0u, 0u, 0u, 0u);
+        ((uint4*)(s_cnt + w * 256))[lane] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t tbase = (uint64_t)tile * TILE;
+    const bool full = tbase + TILE <= a.n;
+    const uint32_t valid = full ? TILE : (uint32_t)(a.n - tbase);
+
+    // ---- load (wave-striped: wave w owns TILE/WAVES consecutive keys) ----
+    KR key[K];
+    VM val[K];
+    const uint64_t wb = tbase + (uint64_t)w * 64u * K + lane;
+    if (full) {
+#pragma unroll
+        for (int i = 0; i < K; ++i) key[i] = key_to_reg<KB>(kin[wb + (uint64_t)i * 64u]);
+        if constexpr (VB > 0) {
+#pragma unroll
+            for (int i = 0; i < K; ++i) val[i] = vin[wb + (uint64_t)i * 64u];
+        }
+        if (first) {
+#pragma unroll
+            for (int i = 0; i < K; ++i) key[i] = tw_in<KB>(key[i], a.tw);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const uint64_t idx = wb + (uint64_t)i * 64u;
+            if (idx < a.n) {
+                KR x = key_to_reg<KB>(kin[idx]);
+                if (first) x = tw_in<KB>(x, a.tw);
+                key[i] = x;
+                if constexpr (VB > 0) val[i] = vin[idx];
+            } else {
+                key[i] = key_filler<KB>();  // sorts last in the tile
+                if constexpr (VB > 0) val[i] = VM{};
+            }
+        }
+    }
+
+    // ---- rank: stable rank of each item among same-digit items of its wave ----
+    const uint32_t sh = 8u * (a.pass & 3u);
+    const uint32_t wsel = a.pass >> 2;
+    uint64_t* M = (uint64_t*)s_big + (size_t)w * 256;
+    uint32_t* C = s_cnt + w * 256;
+    const uint64_t lanebit = 1ull << lane;
+    uint32_t dr[K];  // rank << 8 | digit
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        const uint32_t d = key_digit(key[i], sh, wsel);
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        uint32_t rank;
+        if (__all(d == d0)) {
+            const uint32_t c = lds_ld(C + d0);
+            rank = c + lane;
+            RSA_COMPILER_FENCE();
+            if (lane == 0) lds_st(C + d0, c + 64u);
+        } else {
+            // peer mask = lanes of this wave holding the same digit
+            __hip_atomic_fetch_or(M + d, lanebit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            RSA_COMPILER_FENCE();
+            const uint64_t m = lds_ld(M + d);
+            RSA_COMPILER_FENCE();
+            lds_st(M + d, (uint64_t)0);
+            RSA_COMPILER_FENCE();
+            const uint32_t c = lds_ld(C + d);
+            const uint32_t below = mask_below(m);
+            rank = c + below;
+            RSA_COMPILER_FENCE();
+            if (below == 0u) lds_st(C + d, c + (uint32_t)__builtin_popcountll(m));
+        }
+        RSA_COMPILER_FENCE();
+        dr[i] = (rank << 8) | d;
+    }
+    __syncthreads();
+
+    // ---- per digit: wave prefix, early aggregate publish, tile scan ----
+    Off* lb = (Off*)a.lookback;
+    uint32_t total = 0, incl = 0, pub = 0;
+    Off gbase = 0;
+    if (tid < 256) {
+        gbase = ((const Off*)a.gp)[a.pass * 256u + tid];
+        uint32_t s = 0;
+#pragma unroll
+        for (int ww = 0; ww < WAVES; ++ww) {
+            const uint32_t c = s_cnt[ww * 256 + tid];
+            s_cnt[ww * 256 + tid] = s;
+            s += c;
+        }
+        total = s;
+        pub = total - ((tid == 255u) ? (TILE - valid) : 0u);  // fillers are not keys
+        __hip_atomic_store(lb + (size_t)tile * kRadix + tid,
+                           lb_pack<Off>(tile == 0u ? kLbPrefix : kLbAggregate, epoch, (Off)pub), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        incl = wave_incl_scan<uint32_t>(total);
+        if (lane == 63u) s_wsum[w] = incl;
+    }
+    __syncthreads();
+    if (tid < 256) {
+        uint32_t woff = 0;
+        for (uint32_t ww = 0; ww < w; ++ww) woff += s_wsum[ww];
+        const uint32_t tds = woff + incl - total;  // tile-local start of digit tid
+        Off prefix = 0;
+        if (tile > 0u) {
+            prefix = lookback_prefix<Off>(lb + tid, tile, epoch);
+            __hip_atomic_store(lb + (size_t)tile * kRadix + tid, lb_pack<Off>(kLbPrefix, epoch, prefix + (Off)pub),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_base[tid] = gbase + prefix - (Off)tds;
+#pragma unroll
+        for (int ww = 0; ww < WAVES; ++ww) s_cnt[ww * 256 + tid] += tds;
+    }
+    __syncthreads();
+
+    // ---- stage keys at their tile rank, then write coalesced digit runs ----
+    KM* SK = (KM*)s_big;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        const uint32_t d = dr[i] & 255u;
+        SK[s_cnt[w * 256 + d] + (dr[i] >> 8)] = key_to_mem<KB>(key[i]);
+    }
+    __syncthreads();
+    Off gi[VB > 0 ? K : 1];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const uint32_t pos = (uint32_t)j * THREADS + tid;
+        if (full || pos < valid) {
+            KR k = key_to_reg<KB>(SK[pos]);
+            const Off g = s_base[key_digit(k, sh, wsel)] + (Off)pos;
+            if (last) k = tw_out<KB>(k, a.tw);
+            kout[g] = key_to_mem<KB>(k);
+            if constexpr (VB > 0) gi[j] = g;
+        }
+    }
+    if constexpr (VB > 0) {
+        __syncthreads();
+        VM* SV = (VM*)s_big;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const uint32_t d = dr[i] & 255u;
+            SV[s_cnt[w * 256 + d] + (dr[i] >> 8)] = val[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const uint32_t pos = (uint32_t)j * THREADS + tid;
+            if (full || pos < valid) vout[gi[j]] = SV[pos];
+        }
+    }
+}
+
+}  // namespace rsa

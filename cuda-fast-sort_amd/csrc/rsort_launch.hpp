@@ -1,0 +1,68 @@
+// Host-side op sequence of one sort (the counterpart of
+// onesweep_byte_sort_enqueue, include/rsort/detail/radix_kernel.cuh:336-474):
+//   clear control block -> histogram (+ lookback clear, offsets, skip mask)
+//   -> one onesweep launch per pass -> device-decided copy-back.
+// Every launch goes on the caller's stream; nothing here allocates.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rsort_protocol.hpp"
+#include "rsort_tuning.hpp"
+
+namespace rsa {
+
+struct SortJob {
+    uint32_t kb;          // key bytes 1,2,4,8,16
+    uint32_t vb;          // value bytes as seen by the kernels: 0,1,2,4,8,16
+    bool wide;            // 64-bit offsets and lookback words
+    bool allow_skip;      // skip passes whose digit is uniform
+    TwMasks tw;
+    void* keys;           // caller buffer (in/out)
+    void* vals;           // caller buffer (in/out) or index buffer
+    void* tmp_keys;       // workspace ping-pong
+    void* tmp_vals;
+    Ctrl* ctrl;           // control header, followed by counts then gp
+    void* counts;
+    void* gp;
+    void* lookback;
+    uint64_t lb_bytes;
+    uint64_t n;
+    uint64_t num_tiles;
+    uint32_t passes;
+    int num_cus;
+    hipEvent_t* events;   // optional: one event recorded before and after each launch
+    int max_events;
+    int* n_events;
+};
+
+// One launcher per key width, each in its own translation unit.
+hipError_t launch_core_kb1(const SortJob& j, hipStream_t s);
+hipError_t launch_core_kb2(const SortJob& j, hipStream_t s);
+hipError_t launch_core_kb4(const SortJob& j, hipStream_t s);
+hipError_t launch_core_kb8(const SortJob& j, hipStream_t s);
+hipError_t launch_core_kb16(const SortJob& j, hipStream_t s);
+
+// Helpers living in the host translation unit.
+hipError_t launch_iota(uint32_t* p, uint64_t n, int num_cus, hipStream_t s);
+hipError_t launch_gather(void* dst, const void* src, const uint32_t* idx, uint64_t n, uint32_t rec_bytes,
+                         int num_cus, hipStream_t s);
+hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, int num_cus, hipStream_t s);
+
+inline uint32_t grid_for(uint64_t work, uint32_t per_block, int num_cus, uint32_t blocks_per_cu) {
+    uint64_t g = (work + per_block - 1) / per_block;
+    const uint64_t cap = (uint64_t)(num_cus > 0 ? num_cus : 256) * blocks_per_cu;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (uint32_t)g;
+}
+
+inline void record_event(const SortJob& j, hipStream_t s) {
+    if (j.events && j.n_events && *j.n_events < j.max_events) {
+        (void)hipEventRecord(j.events[*j.n_events], s);
+        ++*j.n_events;
+    }
+}
+
+}  // namespace rsa

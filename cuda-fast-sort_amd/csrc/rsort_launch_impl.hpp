@@ -1,0 +1,100 @@
+// Launcher template instantiated once per key width (sort_kb*.hip).
+#pragma once
+
+#include "rsort_kernels.hpp"
+#include "rsort_launch.hpp"
+
+namespace rsa {
+
+template <int KB, int VB, bool WIDE>
+hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
+    using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
+    using KM = typename KeyTraits<KB>::Mem;
+    using VM = typename ValTraits<VB>::Mem;
+    constexpr TileCfg C = tile_cfg(KB, VB);
+    constexpr uint32_t P = KB;
+
+    record_event(j, s);
+    // 1. control header + global digit counts
+    const uint32_t zero_words = (uint32_t)((sizeof(Ctrl) + (size_t)P * kRadix * sizeof(Off)) / 4u);
+    hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(zero_words, 256, j.num_cus, 1)), dim3(256), 0, s,
+                       (uint32_t*)j.ctrl, zero_words);
+    record_event(j, s);
+
+    // 2. histogram of all passes, lookback clear, offsets, skip mask
+    HistParams h{};
+    h.keys = j.keys;
+    h.counts = j.counts;
+    h.gp = j.gp;
+    h.ctrl = j.ctrl;
+    h.lb_zero = (uint4*)j.lookback;
+    h.lb_zero_units = j.lb_bytes / 16u;
+    h.n = j.n;
+    {
+        const uintptr_t addr = (uintptr_t)j.keys;
+        uint64_t head = ((16u - (addr & 15u)) & 15u) / KB;
+        if (head > j.n) head = j.n;
+        h.head = head;
+    }
+    h.passes = P;
+    h.allow_skip = j.allow_skip ? 1u : 0u;
+    h.tw = j.tw;
+    {
+        constexpr uint32_t keys_per_block_iter = (kHistThreads / 64) * 64u * (16u / KB) * 4u;
+        const uint32_t grid = grid_for(j.n, keys_per_block_iter, j.num_cus, 2);
+        hipLaunchKernelGGL((k_histogram<KB, WIDE>), dim3(grid), dim3(kHistThreads), 0, s, h);
+    }
+    record_event(j, s);
+
+    // 3. passes
+    PassParams pp{};
+    pp.keys_a = j.keys;
+    pp.keys_b = j.tmp_keys;
+    pp.vals_a = j.vals;
+    pp.vals_b = j.tmp_vals;
+    pp.gp = j.gp;
+    pp.lookback = j.lookback;
+    pp.ctrl = j.ctrl;
+    pp.n = j.n;
+    pp.passes = P;
+    pp.tw = j.tw;
+    for (uint32_t p = 0; p < P; ++p) {
+        pp.pass = p;
+        hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, C.threads, C.items>), dim3((uint32_t)j.num_tiles),
+                           dim3(C.threads), 0, s, pp);
+        record_event(j, s);
+    }
+
+    // 4. copy back when the executed pass count is odd (decided on device)
+    if ((P & 1u) || j.allow_skip) {
+        const uint32_t g = grid_for(j.n, 256u * 4u, j.num_cus, 8);
+        hipLaunchKernelGGL((k_copy_back<KM>), dim3(g), dim3(256), 0, s, (const Ctrl*)j.ctrl, P, (KM*)j.keys,
+                           (const KM*)j.tmp_keys, j.n);
+        if constexpr (VB > 0) {
+            hipLaunchKernelGGL((k_copy_back<VM>), dim3(g), dim3(256), 0, s, (const Ctrl*)j.ctrl, P, (VM*)j.vals,
+                               (const VM*)j.tmp_vals, j.n);
+        }
+        record_event(j, s);
+    }
+    return hipGetLastError();
+}
+
+template <int KB, int VB>
+hipError_t launch_core_v(const SortJob& j, hipStream_t s) {
+    return j.wide ? launch_core_t<KB, VB, true>(j, s) : launch_core_t<KB, VB, false>(j, s);
+}
+
+template <int KB>
+hipError_t launch_core(const SortJob& j, hipStream_t s) {
+    switch (j.vb) {
+        case 0: return launch_core_v<KB, 0>(j, s);
+        case 1: return launch_core_v<KB, 1>(j, s);
+        case 2: return launch_core_v<KB, 2>(j, s);
+        case 4: return launch_core_v<KB, 4>(j, s);
+        case 8: return launch_core_v<KB, 8>(j, s);
+        case 16: return launch_core_v<KB, 16>(j, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace rsa

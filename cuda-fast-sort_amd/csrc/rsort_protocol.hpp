@@ -1,0 +1,133 @@
+// Shared host/device definitions of the onesweep protocol: the control block
+// that lives in the caller's workspace, the lookback word layout, and the
+// key-twiddle masks.  Included by the HIP kernels, by the host dispatcher and
+// by the CPU-side protocol simulation test, so it must compile with plain g++.
+//
+// Reference counterparts:
+//   lookback word layout    include/rsort/detail/lookback.cuh:31-58 (flags + epoch + value)
+//   epoch validity rule     README.md:117 ("matching epoch and no flag = not yet published")
+//   twiddles                include/rsort/detail/tuning.cuh:297-397 (radix_traits)
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define RSA_HD __host__ __device__ __forceinline__
+#else
+#define RSA_HD inline
+#endif
+
+namespace rsa {
+
+constexpr uint32_t kRadix = 256;
+constexpr uint32_t kMaxPasses = 16;
+
+// ---------------------------------------------------------------------------
+// Control block.  256 bytes at the start of the control region; it is cleared
+// by the init kernel at the start of every sort, so stale workspace contents
+// never leak into a new sort.
+// ---------------------------------------------------------------------------
+struct Ctrl {
+    uint32_t ticket[kMaxPasses];  // per-pass tile tickets (dynamic tile order)
+    uint32_t hist_done;           // histogram workgroups finished
+    uint32_t skip_mask;           // bit p set: every key has the same digit in pass p
+    uint32_t reserved[46];
+};
+static_assert(sizeof(Ctrl) == 256, "control header must stay 256 bytes");
+
+// ---------------------------------------------------------------------------
+// Lookback words.  One word per (tile, digit).  Flag, epoch and value share
+// one machine word so a single atomic load observes a consistent triple.
+//   32-bit: [31:30] flag  [29] epoch  [28:0] value   (n <= 2^29 - 1)
+//   64-bit: [63:62] flag  [61] epoch  [60:0] value
+// flag 0 = not published, 1 = tile aggregate only, 2 = inclusive prefix.
+// The epoch is the parity of the executed-pass index, so a word left over
+// from the previous executed pass never reads as valid; the whole array is
+// zeroed once per sort (flag 0 is invalid for every epoch).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kLbInvalid = 0;
+constexpr uint32_t kLbAggregate = 1;
+constexpr uint32_t kLbPrefix = 2;
+
+template <typename W>
+struct LbLayout;
+
+template <>
+struct LbLayout<uint32_t> {
+    static constexpr int kFlagShift = 30;
+    static constexpr int kEpochShift = 29;
+    static constexpr uint32_t kValueMask = (1u << 29) - 1u;
+};
+
+template <>
+struct LbLayout<uint64_t> {
+    static constexpr int kFlagShift = 62;
+    static constexpr int kEpochShift = 61;
+    static constexpr uint64_t kValueMask = (1ull << 61) - 1ull;
+};
+
+template <typename W>
+RSA_HD W lb_pack(uint32_t flag, uint32_t epoch, W value) {
+    using L = LbLayout<W>;
+    return (W(flag) << L::kFlagShift) | (W(epoch & 1u) << L::kEpochShift) | (value & L::kValueMask);
+}
+
+template <typename W>
+RSA_HD uint32_t lb_flag(W w) {
+    return uint32_t(w >> LbLayout<W>::kFlagShift) & 3u;
+}
+
+template <typename W>
+RSA_HD uint32_t lb_epoch(W w) {
+    return uint32_t(w >> LbLayout<W>::kEpochShift) & 1u;
+}
+
+template <typename W>
+RSA_HD W lb_value(W w) {
+    return w & LbLayout<W>::kValueMask;
+}
+
+// A word is usable by a pass of epoch `epoch` iff it carries a flag AND the
+// matching epoch.  (The reference tests only the epoch bits,
+// lookback.cuh:200-207; requiring a flag as well removes its tag-word hazard.)
+template <typename W>
+RSA_HD bool lb_ready(W w, uint32_t epoch) {
+    return lb_flag(w) != kLbInvalid && lb_epoch(w) == (epoch & 1u);
+}
+
+// Largest n the 32-bit word can describe: an inclusive prefix may equal n.
+constexpr uint64_t kNarrowMaxN = (1ull << 29) - 1ull;
+
+// ---------------------------------------------------------------------------
+// Key twiddle masks (runtime, wave-uniform).  For a key of W bits held in an
+// unsigned word u:
+//   in : t = u ^ ((sext(msb(u)) & fmask) | smask) ^ dmask
+//   out: u = t' ^ ((~sext(msb(t')) & fmask) | smask),  t' = t ^ dmask
+// with fmask = ones(W) for floating keys, smask = sign bit for signed and
+// floating keys, dmask = ones(W) for descending order.  For 128-bit keys the
+// masks apply per 64-bit half and smask only to the high half.
+// ---------------------------------------------------------------------------
+enum KeyKind : uint32_t { kKindUnsigned = 0, kKindSigned = 1, kKindFloat = 2 };
+
+struct TwMasks {
+    uint64_t fmask;  // per 64-bit half
+    uint64_t smask;  // high half only
+    uint64_t dmask;  // per 64-bit half
+    uint32_t active; // nonzero when the twiddle is not the identity
+    uint32_t pad;
+};
+
+RSA_HD TwMasks make_tw_masks(uint32_t kind, uint32_t key_bytes, bool descending) {
+    const uint32_t bits = key_bytes >= 8 ? 64u : key_bytes * 8u;
+    const uint64_t ones = bits == 64 ? ~0ull : ((1ull << bits) - 1ull);
+    const uint64_t sign = 1ull << (bits - 1u);
+    TwMasks m{};
+    m.fmask = (kind == kKindFloat) ? ones : 0ull;
+    m.smask = (kind == kKindSigned || kind == kKindFloat) ? sign : 0ull;
+    m.dmask = descending ? ones : 0ull;
+    m.active = (m.fmask | m.smask | m.dmask) != 0ull;
+    return m;
+}
+
+}  // namespace rsa

@@ -1,0 +1,198 @@
+"""Python host mirror of rsort's API over the C ABI of librsort_amd.so.
+
+Same four entry points and argument meaning as the reference templates
+(include/rsort/rsort.cuh:13-92):
+
+    temp = sort(d_inout, None, None, n)           # size query (no device work)
+    sort(d_inout, temp, d_workspace, n)           # in-place ascending sort
+    sort_descending(...), sort_pairs(keys, values, temp, ws, n), sort_pairs_descending(...)
+
+Device buffers are torch tensors on the GPU (torch provides memory and
+streams only).  The library is loaded from this source tree; if it is
+missing the import fails loudly -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "librsort_amd.so")
+
+KEY_UNSIGNED, KEY_SIGNED, KEY_FLOAT = 0, 1, 2
+FLAG_NO_PASS_SKIP = 1
+
+SUCCESS = 0
+STATUS_NAMES = {
+    0: "RSORT_SUCCESS",
+    1: "RSORT_ERROR_INVALID_ARGUMENT",
+    2: "RSORT_ERROR_UNSUPPORTED",
+    3: "RSORT_ERROR_WORKSPACE_TOO_SMALL",
+    4: "RSORT_ERROR_DEVICE",
+    5: "RSORT_ERROR_BUSY",
+}
+
+
+class RsortError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [
+        ("key_kind", ctypes.c_uint32),
+        ("key_bytes", ctypes.c_uint32),
+        ("value_bytes", ctypes.c_uint32),
+        ("descending", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("passes", ctypes.c_uint32),
+        ("threads", ctypes.c_uint32),
+        ("items_per_thread", ctypes.c_uint32),
+        ("tile", ctypes.c_uint32),
+        ("num_tiles", ctypes.c_uint64),
+        ("wide", ctypes.c_uint32),
+        ("index_mode", ctypes.c_uint32),
+        ("workspace_bytes", ctypes.c_uint64),
+        ("lookback_bytes", ctypes.c_uint64),
+    ]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load librsort_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with python cuda-fast-sort_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
+        L.rsort_sort.argtypes = [ctypes.POINTER(Desc), vp, vp, vp, ctypes.POINTER(ctypes.c_size_t), u64, vp]
+        L.rsort_sort.restype = i32
+        L.rsort_sort_profiled.argtypes = [ctypes.POINTER(Desc), vp, vp, vp, ctypes.POINTER(ctypes.c_size_t), u64, vp,
+                                          ctypes.POINTER(ctypes.c_float), i32, ctypes.POINTER(i32)]
+        L.rsort_sort_profiled.restype = i32
+        L.rsort_plan.argtypes = [ctypes.POINTER(Desc), u64, ctypes.POINTER(PlanInfo)]
+        L.rsort_plan.restype = i32
+        L.rsort_status_string.argtypes = [i32]
+        L.rsort_status_string.restype = ctypes.c_char_p
+        L.rsort_last_error.argtypes = []
+        L.rsort_last_error.restype = ctypes.c_char_p
+        L.rsort_abi_version.argtypes = []
+        L.rsort_abi_version.restype = i32
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().rsort_last_error().decode()
+
+
+def _check(status: int) -> None:
+    if status != SUCCESS:
+        raise RsortError(status, last_error())
+
+
+@dataclass(frozen=True)
+class KeyType:
+    kind: int
+    nbytes: int
+
+
+def key_type_of(dtype) -> KeyType:
+    """Key kind/width of a torch dtype (radix_traits, tuning.cuh:297-397)."""
+    import torch
+
+    table = {
+        torch.uint8: KeyType(KEY_UNSIGNED, 1), torch.int8: KeyType(KEY_SIGNED, 1),
+        torch.uint16: KeyType(KEY_UNSIGNED, 2), torch.int16: KeyType(KEY_SIGNED, 2),
+        torch.uint32: KeyType(KEY_UNSIGNED, 4), torch.int32: KeyType(KEY_SIGNED, 4),
+        torch.uint64: KeyType(KEY_UNSIGNED, 8), torch.int64: KeyType(KEY_SIGNED, 8),
+        torch.float16: KeyType(KEY_FLOAT, 2), torch.bfloat16: KeyType(KEY_FLOAT, 2),
+        torch.float32: KeyType(KEY_FLOAT, 4), torch.float64: KeyType(KEY_FLOAT, 8),
+    }
+    if dtype not in table:
+        raise TypeError(f"no radix key mapping for {dtype}; pass key_type=KeyType(kind, nbytes)")
+    return table[dtype]
+
+
+def make_desc(key_kind: int, key_bytes: int, value_bytes: int = 0, descending: bool = False, flags: int = 0) -> Desc:
+    return Desc(key_kind, key_bytes, value_bytes, 1 if descending else 0, flags, 0)
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)
+
+
+def sort_raw(desc: Desc, keys_ptr: int | None, vals_ptr: int | None, ws_ptr: int | None, temp_bytes: int | None,
+             n: int, stream=None) -> int:
+    """Direct call of rsort_sort; returns the workspace size (query) or temp_bytes."""
+    tb = ctypes.c_size_t(temp_bytes or 0)
+    _check(lib().rsort_sort(ctypes.byref(desc), keys_ptr, vals_ptr, ws_ptr, ctypes.byref(tb), n, _stream_ptr(stream)))
+    return int(tb.value)
+
+
+def plan(desc: Desc, n: int) -> PlanInfo:
+    out = PlanInfo()
+    _check(lib().rsort_plan(ctypes.byref(desc), n, ctypes.byref(out)))
+    return out
+
+
+def _ptr(t) -> int | None:
+    return None if t is None else int(t.data_ptr())
+
+
+def _vbytes(values) -> int:
+    return 0 if values is None else values.element_size() * (values[0].numel() if values.dim() > 1 else 1)
+
+
+def _sort(keys, values, temp_bytes, workspace, n, stream, descending, key_type, flags):
+    kt = key_type or key_type_of(keys.dtype)
+    desc = make_desc(kt.kind, kt.nbytes, _vbytes(values), descending, flags)
+    return sort_raw(desc, _ptr(keys), _ptr(values), _ptr(workspace), temp_bytes, int(n), stream)
+
+
+def sort(d_inout, temp_bytes, d_workspace, n, stream=None, key_type: KeyType | None = None, flags: int = 0) -> int:
+    """rsort::sort (rsort.cuh:13-30).  d_workspace None -> returns the workspace size."""
+    return _sort(d_inout, None, temp_bytes, d_workspace, n, stream, False, key_type, flags)
+
+
+def sort_descending(d_inout, temp_bytes, d_workspace, n, stream=None, key_type: KeyType | None = None,
+                    flags: int = 0) -> int:
+    """rsort::sort_descending (rsort.cuh:33-50)."""
+    return _sort(d_inout, None, temp_bytes, d_workspace, n, stream, True, key_type, flags)
+
+
+def sort_pairs(d_inout, d_inout_values, temp_bytes, d_workspace, n, stream=None, key_type: KeyType | None = None,
+               flags: int = 0) -> int:
+    """rsort::sort_pairs (rsort.cuh:53-71)."""
+    return _sort(d_inout, d_inout_values, temp_bytes, d_workspace, n, stream, False, key_type, flags)
+
+
+def sort_pairs_descending(d_inout, d_inout_values, temp_bytes, d_workspace, n, stream=None,
+                          key_type: KeyType | None = None, flags: int = 0) -> int:
+    """rsort::sort_pairs_descending (rsort.cuh:74-92)."""
+    return _sort(d_inout, d_inout_values, temp_bytes, d_workspace, n, stream, True, key_type, flags)
+
+
+def sort_profiled(desc: Desc, keys_ptr, vals_ptr, ws_ptr, temp_bytes: int, n: int, stream=None, max_stages: int = 40):
+    """rsort_sort_profiled: sorts, synchronises, returns per-stage device ms."""
+    stage = (ctypes.c_float * max_stages)()
+    cnt = ctypes.c_int(0)
+    tb = ctypes.c_size_t(temp_bytes)
+    _check(lib().rsort_sort_profiled(ctypes.byref(desc), keys_ptr, vals_ptr, ws_ptr, ctypes.byref(tb), n,
+                                     _stream_ptr(stream), stage, max_stages, ctypes.byref(cnt)))
+    return [float(stage[i]) for i in range(cnt.value)]

@@ -1,0 +1,115 @@
+/*
+ * rsort_c.h - C ABI of the MI355X (gfx950) onesweep LSD radix sort.
+ *
+ * Plain pointers and sizes only; no HIP or torch types.  The C++ drop-in
+ * templates in <rsort/rsort.hpp> forward to these entry points, and a
+ * ctypes / cgo / JNI binding can call them directly.
+ *
+ * Reference interface replaced (AwardOfSky/CUDA-Fast-Sort v0.4.0):
+ *   rsort::sort / sort_descending / sort_pairs / sort_pairs_descending
+ *     include/rsort/rsort.cuh:13-92
+ *   size query convention (workspace == NULL -> write *temp_bytes, no device work)
+ *     include/rsort/detail/radix_kernel.cuh:501-507
+ *   in-place result, workspace holds the ping-pong buffer
+ *     include/rsort/detail/radix_kernel.cuh:35-109, 468-473; README.md:100
+ */
+#ifndef RSORT_C_H_
+#define RSORT_C_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSORT_ABI_VERSION 1
+
+typedef enum rsort_status {
+    RSORT_SUCCESS = 0,
+    RSORT_ERROR_INVALID_ARGUMENT = 1, /* null pointer, bad descriptor, misaligned buffer */
+    RSORT_ERROR_UNSUPPORTED = 2,      /* key width / kind / size outside what is built */
+    RSORT_ERROR_WORKSPACE_TOO_SMALL = 3,
+    RSORT_ERROR_DEVICE = 4,           /* a HIP runtime call failed */
+    RSORT_ERROR_BUSY = 5              /* serving queue full / stopped */
+} rsort_status;
+
+/* Key kinds; the key is ordered by its value (radix_traits twiddle,
+ * include/rsort/detail/tuning.cuh:297-397). */
+typedef enum rsort_key_kind {
+    RSORT_KEY_UNSIGNED = 0, /* 1,2,4,8,16 bytes (16 = unsigned __int128) */
+    RSORT_KEY_SIGNED = 1,   /* 1,2,4,8,16 bytes, two's complement */
+    RSORT_KEY_FLOAT = 2     /* 2 (IEEE half), 4, 8 bytes; 16 = long double bit
+                               pattern ordered as a sign-magnitude 128-bit word
+                               with the sign at bit 127, as the reference does
+                               (radix_kernel.cuh:351-356, tuning.cuh:324) */
+} rsort_key_kind;
+
+/* Descriptor flags */
+#define RSORT_FLAG_NO_PASS_SKIP 1u /* always run every pass (no uniform-digit skip) */
+
+typedef struct rsort_desc {
+    uint32_t key_kind;    /* rsort_key_kind */
+    uint32_t key_bytes;   /* 1, 2, 4, 8 or 16 */
+    uint32_t value_bytes; /* 0 = keys only; any width (1,2,4,8,16 native, others by index) */
+    uint32_t descending;  /* 0 ascending, 1 descending (stable on the complemented key) */
+    uint32_t flags;       /* RSORT_FLAG_* */
+    uint32_t reserved;    /* must be 0 */
+} rsort_desc;
+
+/*
+ * Stable LSD radix sort of n keys (and their values) in place.
+ *
+ * Replaces rsort::sort / sort_descending (rsort.cuh:13-50, value_bytes == 0)
+ * and rsort::sort_pairs / sort_pairs_descending (rsort.cuh:53-92).
+ *
+ *   workspace == NULL : writes the required size to *temp_bytes, no device work.
+ *   workspace != NULL : *temp_bytes must be >= that size; enqueues the sort on
+ *                       `stream` (a hipStream_t, NULL = default stream) and
+ *                       returns without synchronising.  keys/values are
+ *                       overwritten with the sorted result.
+ *
+ * keys must be aligned to the key width (16 for 16-byte keys), values to
+ * their width when it is a power of two.  n == 0 and n == 1 are no-ops.
+ * One workspace must not be used by two sorts in flight at the same time.
+ */
+rsort_status rsort_sort(const rsort_desc* desc, void* keys, void* values, void* workspace, size_t* temp_bytes,
+                        uint64_t n, void* stream);
+
+/*
+ * Same as rsort_sort, then synchronises the stream and reports the device
+ * time of each launched stage in milliseconds, in launch order:
+ *   [0] control-block clear, [1] histogram, [2..2+P) one per pass,
+ *   [2+P] copy-back (present when the pass count is odd or passes may be skipped).
+ * Measurement helper for benchmarks; no reference counterpart (the reference
+ * times the whole call with cudaEvents, benchmark/benchmark_sort.cuh:623-634).
+ */
+rsort_status rsort_sort_profiled(const rsort_desc* desc, void* keys, void* values, void* workspace,
+                                 size_t* temp_bytes, uint64_t n, void* stream, float* stage_ms, int max_stages,
+                                 int* n_stages);
+
+/* Plan of a sort: what the host would launch for (desc, n).  Host only. */
+typedef struct rsort_plan_info {
+    uint32_t passes;          /* = key_bytes */
+    uint32_t threads;         /* workgroup size of the pass kernel */
+    uint32_t items_per_thread;
+    uint32_t tile;            /* keys per workgroup = threads * items */
+    uint64_t num_tiles;
+    uint32_t wide;            /* 1: 64-bit offsets / lookback words */
+    uint32_t index_mode;      /* 1: values moved by a 32-bit index + gather */
+    uint64_t workspace_bytes;
+    uint64_t lookback_bytes;
+} rsort_plan_info;
+
+rsort_status rsort_plan(const rsort_desc* desc, uint64_t n, rsort_plan_info* out);
+
+/* Human-readable text of a status, and the last error message of the calling thread. */
+const char* rsort_status_string(rsort_status s);
+const char* rsort_last_error(void);
+int rsort_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RSORT_C_H_ */

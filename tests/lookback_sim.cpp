@@ -1,0 +1,94 @@
+// CPU simulation of the tile-ticket + decoupled lookback protocol of the
+// onesweep pass (rsort_protocol.hpp word layout, k_onesweep's publish /
+// lookback order).  Host threads play workgroups: each takes a ticket, publishes
+// its aggregate, walks back until an inclusive prefix, publishes its own
+// prefix.  Several executed passes reuse one lookback array with the epoch
+// parity and no clearing in between; words start zeroed once per sort.
+// Exit status 0 when every tile's prefix equals the sum of earlier tiles.
+#include <stdint.h>
+#include <stdio.h>
+
+#include <atomic>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "../cuda-fast-sort_amd/csrc/rsort_protocol.hpp"
+
+template <typename W>
+static int run(uint32_t tiles, uint32_t digits, uint32_t passes, uint32_t threads, uint32_t seed) {
+    std::vector<std::atomic<W>> lb((size_t)tiles * digits);
+    for (auto& w : lb) w.store(0, std::memory_order_relaxed);  // cleared once per sort
+    std::mt19937 rng(seed);
+    int errors = 0;
+    for (uint32_t e = 0; e < passes; ++e) {
+        const uint32_t epoch = e & 1u;
+        std::vector<uint32_t> count((size_t)tiles * digits);
+        for (auto& c : count) c = rng() % 300u;
+        std::vector<W> expect((size_t)tiles * digits);
+        for (uint32_t d = 0; d < digits; ++d) {
+            W acc = 0;
+            for (uint32_t t = 0; t < tiles; ++t) {
+                expect[(size_t)t * digits + d] = acc;
+                acc += count[(size_t)t * digits + d];
+            }
+        }
+        std::atomic<uint32_t> ticket{0};
+        std::atomic<int> bad{0};
+        auto worker = [&](uint32_t id) {
+            std::mt19937 r(seed * 7919u + id + e * 131u);
+            for (;;) {
+                const uint32_t tile = ticket.fetch_add(1, std::memory_order_relaxed);
+                if (tile >= tiles) return;
+                if (r() % 8 == 0) std::this_thread::yield();
+                for (uint32_t d = 0; d < digits; ++d) {
+                    const W c = count[(size_t)tile * digits + d];
+                    lb[(size_t)tile * digits + d].store(
+                        rsa::lb_pack<W>(tile == 0 ? rsa::kLbPrefix : rsa::kLbAggregate, epoch, c),
+                        std::memory_order_relaxed);
+                }
+                if (r() % 8 == 0) std::this_thread::yield();
+                for (uint32_t d = 0; d < digits; ++d) {
+                    W prefix = 0;
+                    if (tile > 0) {
+                        int64_t j = (int64_t)tile - 1;
+                        for (;;) {
+                            const W w = lb[(size_t)j * digits + d].load(std::memory_order_relaxed);
+                            if (rsa::lb_ready(w, epoch)) {
+                                prefix += rsa::lb_value(w);
+                                if (rsa::lb_flag(w) == rsa::kLbPrefix) break;
+                                --j;
+                            } else {
+                                std::this_thread::yield();
+                            }
+                        }
+                        lb[(size_t)tile * digits + d].store(
+                            rsa::lb_pack<W>(rsa::kLbPrefix, epoch, prefix + count[(size_t)tile * digits + d]),
+                            std::memory_order_relaxed);
+                    }
+                    if (prefix != expect[(size_t)tile * digits + d]) bad.fetch_add(1);
+                }
+            }
+        };
+        std::vector<std::thread> th;
+        for (uint32_t i = 0; i < threads; ++i) th.emplace_back(worker, i);
+        for (auto& t : th) t.join();
+        errors += bad.load();
+    }
+    return errors;
+}
+
+int main() {
+    int errors = 0;
+    for (uint32_t seed = 1; seed <= 4; ++seed) {
+        errors += run<uint32_t>(300, 64, 5, 8, seed);
+        errors += run<uint64_t>(200, 32, 4, 6, seed + 100);
+    }
+    // the packed word keeps flag, epoch and the largest narrow value apart
+    const uint32_t w = rsa::lb_pack<uint32_t>(rsa::kLbPrefix, 1, (uint32_t)rsa::kNarrowMaxN);
+    if (rsa::lb_flag(w) != rsa::kLbPrefix || rsa::lb_epoch(w) != 1 || rsa::lb_value(w) != rsa::kNarrowMaxN) ++errors;
+    if (rsa::lb_ready(0u, 0) || rsa::lb_ready(0u, 1)) ++errors;  // zeroed words are never ready
+    if (rsa::lb_ready(rsa::lb_pack<uint32_t>(rsa::kLbAggregate, 0, 5), 1)) ++errors;  // stale epoch
+    printf("lookback simulation: %d errors\n", errors);
+    return errors == 0 ? 0 : 1;
+}
