@@ -40,10 +40,11 @@ def gpu_sort(rs, keys: np.ndarray, vals: np.ndarray | None, kind: int, kb: int, 
 
 
 def check(rs, orc, keys, vals, kind, kb, vb, desc, flags=0):
+    keys = np.ascontiguousarray(keys).view(np.uint8).reshape(-1)
+    vals = None if vals is None else np.ascontiguousarray(vals).view(np.uint8).reshape(-1)
     gk, gv = gpu_sort(rs, keys, vals, kind, kb, vb, desc, flags)
     ek, ev = orc.sort(keys, vals, kind, kb, desc, vb=vb if vb else None)
     if not np.array_equal(gk, ek):
-        bad = np.where(gk.reshape(-1, kb).any(axis=1) != ek.reshape(-1, kb).any(axis=1))[0]
         diff = np.where((gk.reshape(-1, kb) != ek.reshape(-1, kb)).any(axis=1))[0]
         raise AssertionError(f"keys differ at {diff.size} positions, first {diff[:8]}")
     if vb:
