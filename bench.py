@@ -29,7 +29,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["kv", "keys"], default="kv")
+    ap.add_argument("--workload", choices=["kv", "keys", "copy"], default="kv",
+                    help="kv: sort_pairs u32/u32; keys: sort u32; copy: device copy of the keys "
+                         "(stream-copy bandwidth, and the known-size kernel that calibrates HBM counters)")
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--descending", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -88,6 +90,24 @@ def run_sort_bench(rs, n, with_values, descending, steps, warmup, seed=1):
     return (t1 - t0) * 1e3 / steps, stages, ok, keys0, vals0
 
 
+def copy_bandwidth(n, steps):
+    """Device-to-device copy of n u32 (read n*4 + write n*4 bytes); returns GB/s."""
+    import torch
+
+    src = gen_u32(n, torch.device("cuda:0"), 3)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    return 2.0 * n * 4 / (ms * 1e-3) / 1e9, ms
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -100,6 +120,12 @@ def main():
     rs.lib()
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     n = 1 << a.log2n
+    if a.workload == "copy":
+        gbs, ms = copy_bandwidth(n, a.steps)
+        print(json.dumps({"metric": "device copy bandwidth", "value": round(gbs, 1), "unit": "GB/s",
+                          "ms_per_step": round(ms, 4), "bytes_per_step": 8 * n, "n_gpus": 1,
+                          "steps": a.steps, "higher_is_better": True}), flush=True)
+        return
     kv = a.workload == "kv"
     kb, vb = 4, (4 if kv else 0)
     passes = 4
@@ -148,6 +174,8 @@ def main():
         "histogram_ms": round(sum(hist_ms) / len(hist_ms), 5),
         "checked": ok,
     }
+    cgbs, _ = copy_bandwidth(n, 5)
+    out["stream_copy_gbs"] = round(cgbs, 1)  # measured device copy bandwidth on this box
     if kv and not a.no_extra:
         kms, kst, kok, _, _ = run_sort_bench(rs, n, False, a.descending, max(3, a.steps // 2), 2, seed=2)
         kdev = sum(sum(s) for s in kst) / len(kst)

@@ -15,8 +15,12 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-OBJ = os.path.join(HERE, "_build")
-LIB = os.path.join(HERE, "librsort_amd.so")
+VARIANTS = {
+    # name: (object dir, library, extra flags)
+    "release": (os.path.join(HERE, "_build"), os.path.join(HERE, "librsort_amd.so"), []),
+    "timing": (os.path.join(HERE, "_build", "timing"), os.path.join(HERE, "librsort_amd_timing.so"),
+               ["-DRSA_PHASE_TIMING"]),
+}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = [
@@ -34,22 +38,24 @@ def sources() -> list[str]:
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def _compile(src: str) -> tuple[str, int, str, float]:
-    obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
+def _compile(job: tuple[str, str, list[str]]) -> tuple[str, int, str, float]:
+    src, objdir, extra = job
+    obj = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
     t0 = time.time()
-    cmd = [HIPCC, *HIP_FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *HIP_FLAGS, *extra, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     out = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
     return obj, r.returncode, out, time.time() - t0
 
 
-def build(jobs: int | None = None, verbose: bool = True) -> str:
-    os.makedirs(OBJ, exist_ok=True)
+def build(jobs: int | None = None, verbose: bool = True, variant: str = "release") -> str:
+    objdir, lib, extra = VARIANTS[variant]
+    os.makedirs(objdir, exist_ok=True)
     srcs = sources()
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
     objs = []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        for obj, rc, out, dt in ex.map(_compile, srcs):
+        for obj, rc, out, dt in ex.map(_compile, [(s_, objdir, extra) for s_ in srcs]):
             if out.strip() and verbose:
                 print(out, file=sys.stderr)
             if rc != 0:
@@ -57,20 +63,21 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
             if verbose:
                 print(f"[build] {os.path.basename(obj)} {dt:.1f}s", flush=True)
             objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs,
            "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print(f"[build] {LIB}", flush=True)
-    return LIB
+        print(f"[build] {lib}", flush=True)
+    return lib
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--variant", choices=sorted(VARIANTS), default="release")
     a = ap.parse_args()
-    build(a.jobs)
+    build(a.jobs, variant=a.variant)

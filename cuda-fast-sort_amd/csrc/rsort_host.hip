@@ -52,6 +52,7 @@ struct Plan {
     TileCfg cfg;
     uint64_t tile, num_tiles;
     size_t off_tmp_keys, off_tmp_vals, off_idx, off_vals_copy, off_ctrl, off_counts, off_gp, off_lb;
+    size_t off_hist_partials;
     size_t lb_bytes, total;
 };
 
@@ -94,6 +95,12 @@ rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
     p->off_ctrl = reserve(&off, sizeof(Ctrl));
     p->off_counts = reserve(&off, (size_t)p->passes * kRadix * off_sz, 4);
     p->off_gp = reserve(&off, (size_t)p->passes * kRadix * off_sz, 4);
+    {
+        uint64_t g = (n + hist_keys_per_block_iter((int)p->kb) - 1) / hist_keys_per_block_iter((int)p->kb);
+        if (g > kHistMaxGrid) g = kHistMaxGrid;
+        if (g < 1) g = 1;
+        p->off_hist_partials = reserve(&off, (size_t)g * p->passes * kRadix * 4u);
+    }
     p->lb_bytes = (size_t)p->num_tiles * kRadix * off_sz;
     p->off_lb = reserve(&off, p->lb_bytes);
     p->total = off + 256;  // slack so any 16-byte-aligned workspace pointer can be rounded up
@@ -111,6 +118,23 @@ int device_cus(int dev) {
         cache[dev] = v;
     }
     return cache[dev];
+}
+
+// Phase-timing slots of debug builds (-DRSA_PHASE_TIMING): 64 copies x 32.
+constexpr int kDbgSlots = 32;
+unsigned long long* g_dbg = nullptr;
+
+unsigned long long* debug_slots() {
+#ifdef RSA_PHASE_TIMING
+    static std::once_flag once;
+    std::call_once(once, [] {
+        if (hipMalloc(&g_dbg, 64 * kDbgSlots * sizeof(unsigned long long)) == hipSuccess)
+            (void)hipMemset(g_dbg, 0, 64 * kDbgSlots * sizeof(unsigned long long));
+        else
+            g_dbg = nullptr;
+    });
+#endif
+    return g_dbg;
 }
 
 hipError_t dispatch_core(const SortJob& j, hipStream_t s) {
@@ -164,12 +188,14 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
     j.ctrl = (Ctrl*)(base + p.off_ctrl);
     j.counts = base + p.off_counts;
     j.gp = base + p.off_gp;
+    j.hist_partials = (uint32_t*)(base + p.off_hist_partials);
     j.lookback = base + p.off_lb;
     j.lb_bytes = p.lb_bytes;
     j.n = n;
     j.num_tiles = p.num_tiles;
     j.passes = p.passes;
     j.num_cus = cus;
+    j.dbg = debug_slots();
     j.events = events;
     j.max_events = max_events;
     j.n_events = n_events;
@@ -296,6 +322,22 @@ const char* rsort_status_string(rsort_status s) {
 }
 
 const char* rsort_last_error(void) { return rsa::g_last_error; }
+
+int rsort_debug_phase_slots(unsigned long long* out, int max_slots, int reset) {
+    unsigned long long* d = rsa::debug_slots();
+    if (!d || !out) return 0;
+    unsigned long long h[64 * rsa::kDbgSlots];
+    if (hipDeviceSynchronize() != hipSuccess) return 0;
+    if (hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    const int k = max_slots < rsa::kDbgSlots ? max_slots : rsa::kDbgSlots;
+    for (int i = 0; i < k; ++i) {
+        unsigned long long v = 0;
+        for (int c = 0; c < 64; ++c) v += h[c * rsa::kDbgSlots + i];
+        out[i] = v;
+    }
+    if (reset) (void)hipMemset(d, 0, sizeof(h));
+    return k;
+}
 
 int rsort_abi_version(void) { return RSORT_ABI_VERSION; }
 

@@ -210,6 +210,46 @@ __device__ __forceinline__ T wave_incl_scan(T x) {
 // program order (the LDS executes one wave's instructions in order).
 #define RSA_COMPILER_FENCE() __atomic_signal_fence(__ATOMIC_SEQ_CST)
 
+// Phase timing (debug builds with -DRSA_PHASE_TIMING): thread 0 of each
+// workgroup adds the wall-clock ticks (100 MHz) of each phase to a slot of
+// the debug buffer; 64 copies of the slots spread the atomics.
+#ifdef RSA_PHASE_TIMING
+#define RSA_TIMER_START(dbgbase)                                                              \
+    unsigned long long* rsa_dbg = (dbgbase) ? (dbgbase) + (blockIdx.x & 63u) * 32u : nullptr; \
+    uint64_t rsa_t = wall_clock64();                                                          \
+    const uint64_t rsa_t0 = rsa_t;
+#define RSA_MARK(slot)                                                      \
+    do {                                                                    \
+        if (threadIdx.x == 0 && rsa_dbg) {                                  \
+            const uint64_t n_ = wall_clock64();                             \
+            atomicAdd(rsa_dbg + (slot), (unsigned long long)(n_ - rsa_t));  \
+            rsa_t = n_;                                                     \
+        }                                                                   \
+    } while (0)
+#define RSA_COUNT(slot, v)                                                        \
+    do {                                                                          \
+        if (rsa_dbg) atomicAdd(rsa_dbg + (slot), (unsigned long long)(v));        \
+    } while (0)
+#define RSA_TOTAL(slot)                                                                   \
+    do {                                                                                  \
+        if (threadIdx.x == 0 && rsa_dbg) {                                                \
+            atomicAdd(rsa_dbg + (slot), (unsigned long long)(wall_clock64() - rsa_t0));   \
+            atomicAdd(rsa_dbg + (slot) + 1, 1ull);                                        \
+        }                                                                                 \
+    } while (0)
+#else
+#define RSA_TIMER_START(dbgbase)
+#define RSA_MARK(slot) \
+    do {               \
+    } while (0)
+#define RSA_COUNT(slot, v) \
+    do {                   \
+    } while (0)
+#define RSA_TOTAL(slot) \
+    do {                \
+    } while (0)
+#endif
+
 template <typename T>
 __device__ __forceinline__ T lds_ld(T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
@@ -224,6 +264,7 @@ __device__ __forceinline__ void lds_st(T* p, T v) {
 // ---------------------------------------------------------------------------
 struct HistParams {
     const void* keys;      // caller's keys (original representation)
+    uint32_t* partials;    // [grid][passes][256] per-workgroup counts
     void* counts;          // [passes][256] Off, zeroed by k_fill_zero
     void* gp;              // [passes][256] Off exclusive digit offsets (out)
     Ctrl* ctrl;
@@ -234,6 +275,7 @@ struct HistParams {
     uint32_t passes;
     uint32_t allow_skip;
     TwMasks tw;
+    unsigned long long* dbg;  // phase-timing slots (RSA_PHASE_TIMING builds), else null
 };
 
 struct PassParams {
@@ -245,9 +287,11 @@ struct PassParams {
     void* lookback;        // [tiles][256] words
     Ctrl* ctrl;
     uint64_t n;
+    uint32_t num_tiles;
     uint32_t pass;
     uint32_t passes;
     TwMasks tw;
+    unsigned long long* dbg;  // phase-timing slots (RSA_PHASE_TIMING builds), else null
 };
 
 // ---------------------------------------------------------------------------
@@ -402,6 +446,7 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     const uint32_t tid = threadIdx.x;
     const uint32_t w = tid >> 6;
     const uint32_t lane = tid & 63u;
+    RSA_TIMER_START(a.dbg);
 
     // clear this workgroup's share of the lookback array (used by the passes)
     {
@@ -476,20 +521,57 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
         }
     }
     __syncthreads();
+    RSA_MARK(16);
 
+    // Two-level fold: every workgroup stores its counts; the last arrival of
+    // each group of kHistGroup workgroups sums the group's rows and adds them
+    // to the global counts (kHistGroup x fewer device atomics); the last group
+    // to finish finalises.
+    {
+        uint32_t* row = a.partials + (size_t)blockIdx.x * (P * 256);
+        for (uint32_t i = tid; i < (uint32_t)(P * 256); i += kHistThreads) {
+            uint32_t s = 0;
+#pragma unroll
+            for (int c = 0; c < COPIES; ++c) s += s_hist[c * P * 256 + i];
+            __hip_atomic_store(row + i, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    const uint32_t group = blockIdx.x / kHistGroup;
+    const uint32_t grp0 = group * kHistGroup;
+    const uint32_t gsize = (gridDim.x - grp0) < (uint32_t)kHistGroup ? (gridDim.x - grp0) : (uint32_t)kHistGroup;
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t prev =
+            __hip_atomic_fetch_add(&a.ctrl->group_done[group], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (prev == gsize - 1u) ? 1u : 0u;
+    }
+    __syncthreads();
+    RSA_MARK(17);
+    if (!s_last) {
+        RSA_TOTAL(20);
+        return;
+    }
     Off* counts = (Off*)a.counts;
     for (uint32_t i = tid; i < (uint32_t)(P * 256); i += kHistThreads) {
         uint32_t s = 0;
-#pragma unroll
-        for (int c = 0; c < COPIES; ++c) s += s_hist[c * P * 256 + i];
+        for (uint32_t m = 0; m < gsize; ++m)
+            s += __hip_atomic_load(a.partials + (size_t)(grp0 + m) * (P * 256) + i, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         if (s) atomicAdd(counts + i, (Off)s);
     }
+    const uint32_t groups = (gridDim.x + kHistGroup - 1) / kHistGroup;
     __threadfence();
     __syncthreads();
-    if (tid == 0) s_last = (atomicAdd(&a.ctrl->hist_done, 1u) == gridDim.x - 1u) ? 1u : 0u;
+    if (tid == 0) {
+        const uint32_t prev = __hip_atomic_fetch_add(&a.ctrl->hist_done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (prev == groups - 1u) ? 1u : 0u;
+    }
     __syncthreads();
-    if (!s_last) return;
-    __threadfence();
+    if (!s_last) {
+        RSA_TOTAL(20);
+        return;
+    }
 
     // last workgroup: exclusive digit offsets per pass + uniform-pass mask
     Off* gp = (Off*)a.gp;
@@ -510,26 +592,34 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
         __syncthreads();
     }
     if (tid == 0) a.ctrl->skip_mask = a.allow_skip ? s_skip : 0u;
+    RSA_MARK(18);
+    RSA_TOTAL(20);
 }
 
 // ---------------------------------------------------------------------------
 // Onesweep pass.
 // ---------------------------------------------------------------------------
+// Lookback windows: first round and maximum (words of one digit column).
+// Measured on MI355X at n = 2^28 (8K-key tiles): windows of 8 words beat
+// adaptive 4..16/32/64 -- every lookback word loaded costs bandwidth on the
+// coherent path, and walks are long enough that 8 is rarely wasted.
+#ifndef RSA_LB_W0
+#define RSA_LB_W0 8
+#endif
+#ifndef RSA_LB_WMAX
+#define RSA_LB_WMAX 8
+#endif
+
+// Lookback words are read and written with agent-scope relaxed atomics
+// (coherent across the XCD L2s without cache maintenance); no other data
+// travels through them, so no fence is needed.
 template <typename W>
-__device__ __forceinline__ W lookback_prefix(W* col, uint32_t tile, uint32_t epoch) {
-    W acc = 0;
-    W* p = col + (size_t)(tile - 1u) * kRadix;
-    while (true) {
-        const W word = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (lb_ready(word, epoch)) {
-            acc += lb_value(word);
-            if (lb_flag(word) == kLbPrefix) break;
-            p -= kRadix;
-        } else {
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    return acc;
+__device__ __forceinline__ W lb_load(const W* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename W>
+__device__ __forceinline__ void lb_store(W* p, W v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <int KB, int VB, bool WIDE, int THREADS, int K>
@@ -552,6 +642,9 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(PassParams a) {
     __shared__ Off s_base[256];
     __shared__ uint32_t s_wsum[4];
     __shared__ uint32_t s_tile;
+#ifdef RSA_PHASE_TIMING
+    __shared__ uint32_t s_dbg[2];
+#endif
 
     const uint32_t passes = a.passes;
     const uint32_t pmask = passes >= 32 ? ~0u : ((1u << passes) - 1u);
@@ -569,6 +662,10 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(PassParams a) {
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t w = tid >> 6;
+    RSA_TIMER_START(a.dbg);
+#ifdef RSA_PHASE_TIMING
+    if (tid < 2) s_dbg[tid] = 0u;
+#endif
 
     if (tid == 0) s_tile = atomicAdd(&a.ctrl->ticket[a.pass], 1u);
     {
@@ -583,18 +680,14 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(PassParams a) {
     const uint64_t tbase = (uint64_t)tile * TILE;
     const bool full = tbase + TILE <= a.n;
     const uint32_t valid = full ? TILE : (uint32_t)(a.n - tbase);
+    RSA_MARK(0);
 
     // ---- load (wave-striped: wave w owns TILE/WAVES consecutive keys) ----
     KR key[K];
-    VM val[K];
     const uint64_t wb = tbase + (uint64_t)w * 64u * K + lane;
     if (full) {
 #pragma unroll
         for (int i = 0; i < K; ++i) key[i] = key_to_reg<KB>(kin[wb + (uint64_t)i * 64u]);
-        if constexpr (VB > 0) {
-#pragma unroll
-            for (int i = 0; i < K; ++i) val[i] = vin[wb + (uint64_t)i * 64u];
-        }
         if (first) {
 #pragma unroll
             for (int i = 0; i < K; ++i) key[i] = tw_in<KB>(key[i], a.tw);
@@ -607,13 +700,16 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(PassParams a) {
                 KR x = key_to_reg<KB>(kin[idx]);
                 if (first) x = tw_in<KB>(x, a.tw);
                 key[i] = x;
-                if constexpr (VB > 0) val[i] = vin[idx];
             } else {
                 key[i] = key_filler<KB>();  // sorts last in the tile
-                if constexpr (VB > 0) val[i] = VM{};
             }
         }
     }
+
+#ifdef RSA_PHASE_TIMING
+    __builtin_amdgcn_s_waitcnt(0);  // attribute the load latency to its own phase
+#endif
+    RSA_MARK(1);
 
     // ---- rank: stable rank of each item among same-digit items of its wave ----
     const uint32_t sh = 8u * (a.pass & 3u);
@@ -650,6 +746,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(PassParams a) {
         dr[i] = (rank << 8) | d;
     }
     __syncthreads();
+    RSA_MARK(2);
 
     // ---- per digit: wave prefix, early aggregate publish, tile scan ----
     Off* lb = (Off*)a.lookback;
@@ -666,37 +763,75 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(PassParams a) {
         }
         total = s;
         pub = total - ((tid == 255u) ? (TILE - valid) : 0u);  // fillers are not keys
-        __hip_atomic_store(lb + (size_t)tile * kRadix + tid,
-                           lb_pack<Off>(tile == 0u ? kLbPrefix : kLbAggregate, epoch, (Off)pub), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        lb_store(lb + (size_t)tile * kRadix + tid,
+                 lb_pack<Off>(tile == 0u ? kLbPrefix : kLbAggregate, epoch, (Off)pub));
         incl = wave_incl_scan<uint32_t>(total);
         if (lane == 63u) s_wsum[w] = incl;
     }
     __syncthreads();
+    RSA_MARK(3);
+    uint32_t tds = 0;
     if (tid < 256) {
         uint32_t woff = 0;
         for (uint32_t ww = 0; ww < w; ++ww) woff += s_wsum[ww];
-        const uint32_t tds = woff + incl - total;  // tile-local start of digit tid
-        Off prefix = 0;
-        if (tile > 0u) {
-            prefix = lookback_prefix<Off>(lb + tid, tile, epoch);
-            __hip_atomic_store(lb + (size_t)tile * kRadix + tid, lb_pack<Off>(kLbPrefix, epoch, prefix + (Off)pub),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s_base[tid] = gbase + prefix - (Off)tds;
+        tds = woff + incl - total;  // tile-local start of digit tid
 #pragma unroll
         for (int ww = 0; ww < WAVES; ++ww) s_cnt[ww * 256 + tid] += tds;
     }
     __syncthreads();
 
-    // ---- stage keys at their tile rank, then write coalesced digit runs ----
+    // ---- stage keys at their tile rank (tile-local: needs no global prefix),
+    //      which frees the key registers for the lookback window ----
     KM* SK = (KM*)s_big;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
         const uint32_t d = dr[i] & 255u;
         SK[s_cnt[w * 256 + d] + (dr[i] >> 8)] = key_to_mem<KB>(key[i]);
     }
+    RSA_MARK(5);
+
+    // ---- decoupled lookback: global prefix of each digit over earlier tiles ----
+    if (tid < 256) {
+        Off prefix = 0;
+        if (tile > 0u) {
+            uint32_t steps = 0, spins = 0;
+            const Off* col = lb + tid;
+            prefix = lookback_walk<Off, RSA_LB_W0, RSA_LB_WMAX>(
+                (int64_t)tile, epoch, [col](int64_t t) { return lb_load(col + t * (int64_t)kRadix); },
+                []() { __builtin_amdgcn_s_sleep(1); }, steps, spins);
+            lb_store(lb + (size_t)tile * kRadix + tid, lb_pack<Off>(kLbPrefix, epoch, prefix + (Off)pub));
+#ifdef RSA_PHASE_TIMING
+            atomicAdd(&s_dbg[0], steps);
+            atomicAdd(&s_dbg[1], spins);
+#endif
+        }
+        s_base[tid] = gbase + prefix - (Off)tds;
+    }
+    // values are needed only after the key scatter: load them now so the
+    // loads overlap the barrier and the key writes
+    VM val[VB > 0 ? K : 1];
+    if constexpr (VB > 0) {
+        if (full) {
+#pragma unroll
+            for (int i = 0; i < K; ++i) val[i] = vin[wb + (uint64_t)i * 64u];
+        } else {
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+                const uint64_t idx = wb + (uint64_t)i * 64u;
+                val[i] = idx < a.n ? vin[idx] : VM{};
+            }
+        }
+    }
     __syncthreads();
+    RSA_MARK(4);
+#ifdef RSA_PHASE_TIMING
+    if (tid == 0) {
+        RSA_COUNT(8, s_dbg[0]);
+        RSA_COUNT(9, s_dbg[1]);
+    }
+#endif
+
+    // ---- write coalesced digit runs of the staged keys ----
     Off gi[VB > 0 ? K : 1];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
@@ -709,6 +844,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(PassParams a) {
             if constexpr (VB > 0) gi[j] = g;
         }
     }
+    RSA_MARK(6);
     if constexpr (VB > 0) {
         __syncthreads();
         VM* SV = (VM*)s_big;
@@ -723,7 +859,9 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(PassParams a) {
             const uint32_t pos = (uint32_t)j * THREADS + tid;
             if (full || pos < valid) vout[gi[j]] = SV[pos];
         }
+        RSA_MARK(7);
     }
+    RSA_TOTAL(10);
 }
 
 }  // namespace rsa

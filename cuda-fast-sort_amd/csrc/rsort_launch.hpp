@@ -26,12 +26,14 @@ struct SortJob {
     Ctrl* ctrl;           // control header, followed by counts then gp
     void* counts;
     void* gp;
+    uint32_t* hist_partials;    // [hist grid][passes][256]
     void* lookback;
     uint64_t lb_bytes;
     uint64_t n;
     uint64_t num_tiles;
     uint32_t passes;
     int num_cus;
+    unsigned long long* dbg;  // phase-timing slots (RSA_PHASE_TIMING builds)
     hipEvent_t* events;   // optional: one event recorded before and after each launch
     int max_events;
     int* n_events;

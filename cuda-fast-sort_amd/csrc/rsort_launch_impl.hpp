@@ -22,8 +22,12 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     record_event(j, s);
 
     // 2. histogram of all passes, lookback clear, offsets, skip mask
+    const uint32_t hgrid = grid_for(j.n, hist_keys_per_block_iter(KB), j.num_cus, 2) < kHistMaxGrid
+                               ? grid_for(j.n, hist_keys_per_block_iter(KB), j.num_cus, 2)
+                               : kHistMaxGrid;
     HistParams h{};
     h.keys = j.keys;
+    h.partials = j.hist_partials;
     h.counts = j.counts;
     h.gp = j.gp;
     h.ctrl = j.ctrl;
@@ -39,11 +43,8 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     h.passes = P;
     h.allow_skip = j.allow_skip ? 1u : 0u;
     h.tw = j.tw;
-    {
-        constexpr uint32_t keys_per_block_iter = (kHistThreads / 64) * 64u * (16u / KB) * 4u;
-        const uint32_t grid = grid_for(j.n, keys_per_block_iter, j.num_cus, 2);
-        hipLaunchKernelGGL((k_histogram<KB, WIDE>), dim3(grid), dim3(kHistThreads), 0, s, h);
-    }
+    h.dbg = j.dbg;
+    hipLaunchKernelGGL((k_histogram<KB, WIDE>), dim3(hgrid), dim3(kHistThreads), 0, s, h);
     record_event(j, s);
 
     // 3. passes
@@ -56,8 +57,10 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     pp.lookback = j.lookback;
     pp.ctrl = j.ctrl;
     pp.n = j.n;
+    pp.num_tiles = (uint32_t)j.num_tiles;
     pp.passes = P;
     pp.tw = j.tw;
+    pp.dbg = j.dbg;
     for (uint32_t p = 0; p < P; ++p) {
         pp.pass = p;
         hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, C.threads, C.items>), dim3((uint32_t)j.num_tiles),

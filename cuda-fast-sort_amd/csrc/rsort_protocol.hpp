@@ -30,9 +30,10 @@ constexpr uint32_t kMaxPasses = 16;
 // ---------------------------------------------------------------------------
 struct Ctrl {
     uint32_t ticket[kMaxPasses];  // per-pass tile tickets (dynamic tile order)
-    uint32_t hist_done;           // histogram workgroups finished
+    uint32_t hist_done;           // histogram workgroup groups finished
     uint32_t skip_mask;           // bit p set: every key has the same digit in pass p
-    uint32_t reserved[46];
+    uint32_t group_done[32];      // histogram workgroups finished per group
+    uint32_t reserved[14];
 };
 static_assert(sizeof(Ctrl) == 256, "control header must stay 256 bytes");
 
@@ -98,6 +99,55 @@ RSA_HD bool lb_ready(W w, uint32_t epoch) {
 
 // Largest n the 32-bit word can describe: an inclusive prefix may equal n.
 constexpr uint64_t kNarrowMaxN = (1ull << 29) - 1ull;
+
+// Decoupled lookback for one digit of `tile` (tile >= 1): sum the published
+// values of the preceding tiles, walking back until an inclusive prefix.
+// Each round issues the loads of `win` predecessors at once (independent
+// loads, one round trip) and consumes them in order.  The window starts at
+// W0 words and doubles up to WMAX while every word read is an aggregate, so
+// short walks (the steady state) read few words and the long walks at the
+// start of a pass (every resident tile starts at once) take few rounds; a
+// not-ready word ends the round and is polled again after `backoff()`.
+// Tile 0 always publishes an inclusive prefix, so the walk never passes it.
+// `load(t)` returns the word of tile t.  Shared by the pass kernel and the
+// host-thread protocol simulation.
+template <typename W, int W0, int WMAX, typename Load, typename Backoff>
+RSA_HD W lookback_walk(int64_t tile, uint32_t epoch, Load&& load, Backoff&& backoff, uint32_t& steps,
+                       uint32_t& spins) {
+    W acc = 0;
+    int64_t j = tile - 1;  // next predecessor to consume
+    int win = W0;
+    while (true) {
+        W word[WMAX];
+#pragma unroll
+        for (int k = 0; k < WMAX; ++k) word[k] = (k < win && (j - k) >= 0) ? load(j - k) : (W)0;
+        int used = win;
+        bool done = false;
+#pragma unroll
+        for (int k = 0; k < WMAX; ++k) {
+            if (k < win && used == win && !done) {
+                if (!lb_ready(word[k], epoch)) {
+                    used = k;
+                } else {
+                    acc += lb_value(word[k]);
+                    if (lb_flag(word[k]) == kLbPrefix) {
+                        done = true;
+                        used = k + 1;
+                    }
+                }
+            }
+        }
+        steps += (uint32_t)used;
+        if (done) return acc;
+        j -= used;
+        if (used < win) {
+            ++spins;
+            backoff();
+        } else {
+            win = win * 2 < WMAX ? win * 2 : WMAX;
+        }
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Key twiddle masks (runtime, wave-uniform).  For a key of W bits held in an

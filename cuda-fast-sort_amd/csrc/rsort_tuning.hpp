@@ -13,6 +13,18 @@
 namespace rsa {
 
 constexpr int kHistThreads = 512;
+// Histogram workgroups fold their counts in groups of this many (the last
+// arrival of a group sums the group's rows) before touching global atomics.
+constexpr int kHistGroup = 16;
+// Upper bound of the histogram grid (32 groups, the arrival counters that fit
+// in the control header).
+constexpr uint32_t kHistMaxGrid = 32 * kHistGroup;
+
+// Histogram grid for n keys of key_bytes: up to 2 workgroups per CU, each
+// with at least 4 loads of 16 bytes per lane.
+RSA_HD constexpr uint32_t hist_keys_per_block_iter(int key_bytes) {
+    return (uint32_t)(kHistThreads / 64) * 64u * (16u / (uint32_t)key_bytes) * 4u;
+}
 
 // LDS copies of the per-pass digit counters in the histogram kernel: one per
 // wave while it fits in 64 KiB, otherwise shared by pairs of waves.
@@ -26,12 +38,25 @@ struct TileCfg {
     uint32_t items;
 };
 
+#ifndef RSA_THREADS_4_0
+#define RSA_THREADS_4_0 512
+#endif
+#ifndef RSA_ITEMS_4_0
+#define RSA_ITEMS_4_0 16
+#endif
+#ifndef RSA_THREADS_4_4
+#define RSA_THREADS_4_4 512
+#endif
+#ifndef RSA_ITEMS_4_4
+#define RSA_ITEMS_4_4 16
+#endif
+
 // key_bytes in {1,2,4,8,16}; val_bytes in {0,1,2,4,8,16}
 RSA_HD constexpr TileCfg tile_cfg(int key_bytes, int val_bytes) {
     const int wide = key_bytes > val_bytes ? key_bytes : val_bytes;
     if (key_bytes <= 4) {
-        if (val_bytes == 0) return TileCfg{512, 16};
-        if (wide <= 4) return TileCfg{512, 16};
+        if (val_bytes == 0) return TileCfg{RSA_THREADS_4_0, RSA_ITEMS_4_0};
+        if (wide <= 4) return TileCfg{RSA_THREADS_4_4, RSA_ITEMS_4_4};
         if (wide == 8) return TileCfg{512, 12};
         return TileCfg{512, 8};
     }

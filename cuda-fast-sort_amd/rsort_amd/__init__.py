@@ -18,7 +18,8 @@ import os
 from dataclasses import dataclass
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "librsort_amd.so")
+# RSORT_AMD_LIB selects another in-tree build (e.g. librsort_amd_timing.so).
+LIB_PATH = os.path.join(PKG_DIR, os.environ.get("RSORT_AMD_LIB", "librsort_amd.so"))
 
 KEY_UNSIGNED, KEY_SIGNED, KEY_FLOAT = 0, 1, 2
 FLAG_NO_PASS_SKIP = 1
@@ -89,6 +90,8 @@ def lib() -> ctypes.CDLL:
         L.rsort_last_error.restype = ctypes.c_char_p
         L.rsort_abi_version.argtypes = []
         L.rsort_abi_version.restype = i32
+        L.rsort_debug_phase_slots.argtypes = [ctypes.POINTER(ctypes.c_ulonglong), i32, i32]
+        L.rsort_debug_phase_slots.restype = i32
         _lib = L
     return _lib
 
@@ -196,3 +199,10 @@ def sort_profiled(desc: Desc, keys_ptr, vals_ptr, ws_ptr, temp_bytes: int, n: in
     _check(lib().rsort_sort_profiled(ctypes.byref(desc), keys_ptr, vals_ptr, ws_ptr, ctypes.byref(tb), n,
                                      _stream_ptr(stream), stage, max_stages, ctypes.byref(cnt)))
     return [float(stage[i]) for i in range(cnt.value)]
+
+
+def debug_phase_slots(reset: bool = True) -> list[int]:
+    """Per-phase tick sums of a -DRSA_PHASE_TIMING build (empty otherwise)."""
+    buf = (ctypes.c_ulonglong * 32)()
+    k = lib().rsort_debug_phase_slots(buf, 32, 1 if reset else 0)
+    return [int(buf[i]) for i in range(k)]

@@ -103,6 +103,18 @@ typedef struct rsort_plan_info {
 
 rsort_status rsort_plan(const rsort_desc* desc, uint64_t n, rsort_plan_info* out);
 
+/*
+ * Debug builds compiled with -DRSA_PHASE_TIMING accumulate per-phase wall-clock
+ * ticks (100 MHz) of the kernels; this copies the summed slots to `out` and
+ * returns how many were written (0 in normal builds).  Slots of the pass
+ * kernel: 0 ticket, 1 load, 2 rank, 3 scan + aggregate publish, 4 lookback,
+ * 5 stage, 6 key scatter, 7 value stage + scatter, 8 lookback word loads,
+ * 9 lookback not-ready polls, 10 total, 11 tiles; histogram: 16 count loop,
+ * 17 reduction + global adds, 18 finalize, 20 total, 21 workgroups.
+ * No reference counterpart (the reference tunes from SASS listings).
+ */
+int rsort_debug_phase_slots(unsigned long long* out, int max_slots, int reset);
+
 /* Human-readable text of a status, and the last error message of the calling thread. */
 const char* rsort_status_string(rsort_status s);
 const char* rsort_last_error(void);

@@ -110,4 +110,4 @@ def test_lookback_protocol_simulation(tmp_path):
     subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", src, "-o", str(exe)], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "0 errors" in r.stdout
+    assert ": 0 errors" in r.stdout
