@@ -10,6 +10,7 @@
 #include <stdint.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <mutex>
@@ -45,7 +46,19 @@ size_t reserve(size_t* off, size_t bytes, size_t align = 256) {
 
 bool is_pow2_width(uint32_t b) { return b == 1 || b == 2 || b == 4 || b == 8 || b == 16; }
 
+// Pass kernel choice: RSORT_AMD_PASS_KERNEL=tile|chunk (default tile: on
+// MI355X the per-tile kernel measured faster -- the chunked kernel's single
+// workgroup per CU cannot hide its ranking latency).
+uint32_t pass_kernel_choice() {
+    static const uint32_t v = [] {
+        const char* e = getenv("RSORT_AMD_PASS_KERNEL");
+        return (e && strcmp(e, "chunk") == 0) ? (uint32_t)kPassChunk : (uint32_t)kPassTile;
+    }();
+    return v;
+}
+
 struct Plan {
+    uint32_t pass_kernel;
     uint32_t kind, kb, vb_user, vb_core;
     bool desc, allow_skip, index_mode, wide;
     uint32_t passes;
@@ -79,7 +92,13 @@ rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
         return fail(RSORT_ERROR_UNSUPPORTED, "values of %u bytes need n < 2^32", d->value_bytes);
     p->wide = n > kNarrowMaxN;
     p->passes = p->kb;
-    p->cfg = tile_cfg((int)p->kb, (int)p->vb_core);
+    p->pass_kernel = pass_kernel_choice();
+    if (p->pass_kernel == kPassChunk) {
+        const ChunkCfg cc = chunk_cfg((int)p->kb, (int)p->vb_core);
+        p->cfg = TileCfg{cc.threads, cc.ipt * cc.subs};
+    } else {
+        p->cfg = tile_cfg((int)p->kb, (int)p->vb_core);
+    }
     p->tile = (uint64_t)p->cfg.threads * p->cfg.items;
     p->num_tiles = n ? (n + p->tile - 1) / p->tile : 0;
     if (p->num_tiles > 0x7FFFFFFFull) return fail(RSORT_ERROR_UNSUPPORTED, "n too large");
@@ -176,6 +195,7 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
 
     uint8_t* base = (uint8_t*)(((uintptr_t)workspace + 255u) & ~(uintptr_t)255u);
     SortJob j{};
+    j.pass_kernel = p.pass_kernel;
     j.kb = p.kb;
     j.vb = p.vb_core;
     j.wide = p.wide;

@@ -268,8 +268,6 @@ struct HistParams {
     void* counts;          // [passes][256] Off, zeroed by k_fill_zero
     void* gp;              // [passes][256] Off exclusive digit offsets (out)
     Ctrl* ctrl;
-    uint4* lb_zero;        // lookback array to clear (16-byte units)
-    uint64_t lb_zero_units;
     uint64_t n;
     uint64_t head;         // keys before the first 16-byte aligned group
     uint32_t passes;
@@ -299,6 +297,13 @@ struct PassParams {
 // ---------------------------------------------------------------------------
 static __global__ void k_fill_zero(uint32_t* __restrict__ p, uint32_t words) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Clears the lookback array (16-byte units) once per sort.
+static __global__ void k_fill_zero16(uint4* __restrict__ p, uint64_t units) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < units; i += stride)
+        p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // Odd number of executed passes leaves the result in the workspace buffer:
@@ -436,7 +441,7 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     constexpr int WPC = WAVES / COPIES;
     constexpr int V = HistVec<KB>::V;
     constexpr uint32_t G = 64u * V;  // keys per wave group
-    constexpr int UNROLL = 4;
+    constexpr int UNROLL = 4;  // 16-byte loads per lane per batch
 
     __shared__ uint32_t s_hist[COPIES * P * 256];
     __shared__ Off s_wsum[4];
@@ -448,13 +453,6 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     const uint32_t lane = tid & 63u;
     RSA_TIMER_START(a.dbg);
 
-    // clear this workgroup's share of the lookback array (used by the passes)
-    {
-        const uint64_t per = (a.lb_zero_units + gridDim.x - 1) / gridDim.x;
-        const uint64_t b0 = per * blockIdx.x;
-        const uint64_t b1 = (b0 + per < a.lb_zero_units) ? b0 + per : a.lb_zero_units;
-        for (uint64_t i = b0 + tid; i < b1; i += kHistThreads) a.lb_zero[i] = make_uint4(0u, 0u, 0u, 0u);
-    }
     for (uint32_t i = tid; i < (uint32_t)(COPIES * P * 256); i += kHistThreads) s_hist[i] = 0u;
     if (tid == 0) s_skip = 0u;
     __syncthreads();
@@ -468,33 +466,11 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     const uint64_t g1 = body * (blockIdx.x + 1) / gridDim.x;
     const uint4* base = (const uint4*)(keys + a.head);
 
-    uint64_t g = g0 + w;
-    for (; g + (UNROLL - 1) * WAVES < g1; g += UNROLL * WAVES) {
-        uint4 q[UNROLL];
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) q[u] = base[(g + (uint64_t)u * WAVES) * 64 + lane];
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-            Reg k[V];
-            HistVec<KB>::unpack(q[u], k);
-            if (twid) {
-#pragma unroll
-                for (int j = 0; j < V; ++j) k[j] = tw_in<KB>(k[j], a.tw);
-            }
-            Reg o = k[0], an = k[0];
-#pragma unroll
-            for (int j = 1; j < V; ++j) {
-                o = reg_or(o, k[j]);
-                an = reg_and(an, k[j]);
-            }
-            const Reg o0 = rfl(o);
-            const Reg v = reg_or(reg_xor(o, an), reg_xor(o, o0));
-            hist_count_pass<KB, 0, V>(H, k, v, o0);
-        }
-    }
-    for (; g < g1; g += WAVES) {
+    // Software-pipelined: the next batch of UNROLL loads is in flight while
+    // the current one is counted.
+    auto count_group = [&](const uint4& q) {
         Reg k[V];
-        HistVec<KB>::unpack(base[g * 64 + lane], k);
+        HistVec<KB>::unpack(q, k);
         if (twid) {
 #pragma unroll
             for (int j = 0; j < V; ++j) k[j] = tw_in<KB>(k[j], a.tw);
@@ -508,7 +484,29 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
         const Reg o0 = rfl(o);
         const Reg v = reg_or(reg_xor(o, an), reg_xor(o, o0));
         hist_count_pass<KB, 0, V>(H, k, v, o0);
+    };
+    constexpr uint64_t STEP = (uint64_t)UNROLL * WAVES;
+    uint64_t g = g0 + w;
+    if (g + (UNROLL - 1) * WAVES < g1) {
+        uint4 cur[UNROLL], nxt[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) cur[u] = base[(g + (uint64_t)u * WAVES) * 64 + lane];
+        while (true) {
+            const uint64_t gn = g + STEP;
+            const bool more = gn + (UNROLL - 1) * WAVES < g1;
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) nxt[u] = base[(gn + (uint64_t)u * WAVES) * 64 + lane];
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) count_group(cur[u]);
+            g = gn;
+            if (!more) break;
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) cur[u] = nxt[u];
+        }
     }
+    for (; g < g1; g += WAVES) count_group(base[g * 64 + lane]);
     // head (before the aligned body) and tail (after it): last workgroup
     if (blockIdx.x == gridDim.x - 1) {
         const uint64_t tail0 = a.head + body * G;
@@ -539,11 +537,15 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     const uint32_t group = blockIdx.x / kHistGroup;
     const uint32_t grp0 = group * kHistGroup;
     const uint32_t gsize = (gridDim.x - grp0) < (uint32_t)kHistGroup ? (gridDim.x - grp0) : (uint32_t)kHistGroup;
-    __threadfence();
+    // The rows were written with coherent (write-through) atomic stores, so
+    // waiting for their completion orders them before the arrival count; an
+    // agent-scope release here would write back the whole XCD L2 and, with
+    // every workgroup of the XCD arriving together, serialise on it.
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0) {
         const uint32_t prev =
-            __hip_atomic_fetch_add(&a.ctrl->group_done[group], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&a.ctrl->group_done[group], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_last = (prev == gsize - 1u) ? 1u : 0u;
     }
     __syncthreads();
@@ -561,10 +563,10 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
         if (s) atomicAdd(counts + i, (Off)s);
     }
     const uint32_t groups = (gridDim.x + kHistGroup - 1) / kHistGroup;
-    __threadfence();
+    __builtin_amdgcn_s_waitcnt(0);  // device atomics on the counts have completed
     __syncthreads();
     if (tid == 0) {
-        const uint32_t prev = __hip_atomic_fetch_add(&a.ctrl->hist_done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t prev = __hip_atomic_fetch_add(&a.ctrl->hist_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_last = (prev == groups - 1u) ? 1u : 0u;
     }
     __syncthreads();
@@ -574,6 +576,7 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     }
 
     // last workgroup: exclusive digit offsets per pass + uniform-pass mask
+    // (counts are read with coherent atomic loads)
     Off* gp = (Off*)a.gp;
     for (int p = 0; p < P; ++p) {
         Off c = 0, incl = 0;

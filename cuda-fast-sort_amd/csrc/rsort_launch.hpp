@@ -14,6 +14,7 @@
 namespace rsa {
 
 struct SortJob {
+    uint32_t pass_kernel; // PassKernel
     uint32_t kb;          // key bytes 1,2,4,8,16
     uint32_t vb;          // value bytes as seen by the kernels: 0,1,2,4,8,16
     bool wide;            // 64-bit offsets and lookback words

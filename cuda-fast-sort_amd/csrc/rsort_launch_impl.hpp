@@ -3,6 +3,7 @@
 
 #include "rsort_kernels.hpp"
 #include "rsort_launch.hpp"
+#include "rsort_onesweep_p.hpp"
 
 namespace rsa {
 
@@ -19,20 +20,20 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     const uint32_t zero_words = (uint32_t)((sizeof(Ctrl) + (size_t)P * kRadix * sizeof(Off)) / 4u);
     hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(zero_words, 256, j.num_cus, 1)), dim3(256), 0, s,
                        (uint32_t*)j.ctrl, zero_words);
+    // lookback words start at flag 0 (never ready) for every epoch
+    hipLaunchKernelGGL(k_fill_zero16, dim3(grid_for(j.lb_bytes / 16u, 256u * 8u, j.num_cus, 4)), dim3(256), 0, s,
+                       (uint4*)j.lookback, (uint64_t)(j.lb_bytes / 16u));
     record_event(j, s);
 
     // 2. histogram of all passes, lookback clear, offsets, skip mask
-    const uint32_t hgrid = grid_for(j.n, hist_keys_per_block_iter(KB), j.num_cus, 2) < kHistMaxGrid
-                               ? grid_for(j.n, hist_keys_per_block_iter(KB), j.num_cus, 2)
-                               : kHistMaxGrid;
+    const uint32_t hg = grid_for(j.n, hist_keys_per_block_iter(KB), j.num_cus, hist_blocks_per_cu(KB));
+    const uint32_t hgrid = hg < kHistMaxGrid ? hg : kHistMaxGrid;
     HistParams h{};
     h.keys = j.keys;
     h.partials = j.hist_partials;
     h.counts = j.counts;
     h.gp = j.gp;
     h.ctrl = j.ctrl;
-    h.lb_zero = (uint4*)j.lookback;
-    h.lb_zero_units = j.lb_bytes / 16u;
     h.n = j.n;
     {
         const uintptr_t addr = (uintptr_t)j.keys;
@@ -61,10 +62,18 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     pp.passes = P;
     pp.tw = j.tw;
     pp.dbg = j.dbg;
+    constexpr ChunkCfg CC = chunk_cfg(KB, VB);
     for (uint32_t p = 0; p < P; ++p) {
         pp.pass = p;
-        hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, C.threads, C.items>), dim3((uint32_t)j.num_tiles),
-                           dim3(C.threads), 0, s, pp);
+        if (j.pass_kernel == kPassChunk) {
+            // persistent: one workgroup per CU (its LDS allows no second)
+            const uint32_t grid = j.num_tiles < (uint64_t)j.num_cus ? (uint32_t)j.num_tiles : (uint32_t)j.num_cus;
+            hipLaunchKernelGGL((k_onesweep_p<KB, VB, WIDE, CC.threads, CC.ipt, CC.subs>), dim3(grid),
+                               dim3(CC.threads), 0, s, pp);
+        } else {
+            hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, C.threads, C.items>), dim3((uint32_t)j.num_tiles),
+                               dim3(C.threads), 0, s, pp);
+        }
         record_event(j, s);
     }
 

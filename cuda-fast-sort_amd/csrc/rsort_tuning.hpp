@@ -15,13 +15,13 @@ namespace rsa {
 constexpr int kHistThreads = 512;
 // Histogram workgroups fold their counts in groups of this many (the last
 // arrival of a group sums the group's rows) before touching global atomics.
-constexpr int kHistGroup = 16;
+constexpr int kHistGroup = 32;
 // Upper bound of the histogram grid (32 groups, the arrival counters that fit
 // in the control header).
 constexpr uint32_t kHistMaxGrid = 32 * kHistGroup;
 
-// Histogram grid for n keys of key_bytes: up to 2 workgroups per CU, each
-// with at least 4 loads of 16 bytes per lane.
+// Histogram grid for n keys of key_bytes: each workgroup gets at least 4
+// loads of 16 bytes per lane.
 RSA_HD constexpr uint32_t hist_keys_per_block_iter(int key_bytes) {
     return (uint32_t)(kHistThreads / 64) * 64u * (16u / (uint32_t)key_bytes) * 4u;
 }
@@ -33,10 +33,36 @@ RSA_HD constexpr int hist_copies(int key_bytes) {
                                                                    : (64 * 1024) / (key_bytes * 1024);
 }
 
+// Histogram workgroups per CU: as many as the LDS counters allow, up to 4.
+RSA_HD constexpr uint32_t hist_blocks_per_cu(int key_bytes) {
+    const uint32_t lds = (uint32_t)hist_copies(key_bytes) * (uint32_t)key_bytes * 1024u;
+    const uint32_t b = (160u * 1024u - 1024u) / lds;
+    return b < 1u ? 1u : (b > 4u ? 4u : b);
+}
+
 struct TileCfg {
     uint32_t threads;
     uint32_t items;
 };
+
+// Persistent chunked pass (k_onesweep_p): 1024 threads, chunks of SUBS
+// sub-tiles of threads*ipt keys; every sub-tile's keys (or values) are staged
+// at once in 96 KiB of LDS.
+struct ChunkCfg {
+    uint32_t threads;
+    uint32_t ipt;   // items per thread per sub-tile
+    uint32_t subs;  // sub-tiles per chunk
+};
+
+// Which pass kernel a sort uses: one workgroup per tile (k_onesweep) or the
+// persistent chunked kernel (k_onesweep_p).
+enum PassKernel : uint32_t { kPassTile = 0, kPassChunk = 1 };
+
+RSA_HD constexpr ChunkCfg chunk_cfg(int key_bytes, int val_bytes) {
+    const int wide = key_bytes > val_bytes ? key_bytes : val_bytes;
+    const uint32_t sub = wide <= 4 ? 8192u : (wide == 8 ? 4096u : 2048u);
+    return ChunkCfg{1024u, sub / 1024u, 3u};
+}
 
 #ifndef RSA_THREADS_4_0
 #define RSA_THREADS_4_0 512
