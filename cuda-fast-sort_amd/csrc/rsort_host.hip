@@ -69,7 +69,7 @@ struct Plan {
     size_t lb_bytes, total;
 };
 
-rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
+rsort_status check_desc(const rsort_desc* d) {
     if (!d) return fail(RSORT_ERROR_INVALID_ARGUMENT, "descriptor is NULL");
     if (d->reserved != 0) return fail(RSORT_ERROR_INVALID_ARGUMENT, "descriptor reserved field must be 0");
     if (d->key_kind > RSORT_KEY_FLOAT) return fail(RSORT_ERROR_INVALID_ARGUMENT, "unknown key kind %u", d->key_kind);
@@ -79,7 +79,12 @@ rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
         return fail(RSORT_ERROR_UNSUPPORTED, "no 1-byte floating key type");
     if (d->descending > 1) return fail(RSORT_ERROR_INVALID_ARGUMENT, "descending must be 0 or 1");
     if (d->flags & ~RSORT_FLAG_NO_PASS_SKIP) return fail(RSORT_ERROR_INVALID_ARGUMENT, "unknown flags 0x%x", d->flags);
+    return RSORT_SUCCESS;
+}
 
+rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
+    const rsort_status st = check_desc(d);
+    if (st != RSORT_SUCCESS) return st;
     memset(p, 0, sizeof(*p));
     p->kind = d->key_kind;
     p->kb = d->key_bytes;
@@ -123,6 +128,71 @@ rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
     p->lb_bytes = (size_t)p->num_tiles * kRadix * off_sz;
     p->off_lb = reserve(&off, p->lb_bytes);
     p->total = off + 256;  // slack so any 16-byte-aligned workspace pointer can be rounded up
+    return RSORT_SUCCESS;
+}
+
+// Workspace of a segmented sort (no reference counterpart).  The long-segment
+// tables are sized for the worst case of disjoint segments: at most
+// n / (LDS capacity + 1) segments are too long for the LDS bins, and their
+// onesweep tiles number at most n / tile plus one partial tile each.
+struct SegPlan {
+    uint32_t kind, kb, vb_user, vb_core;
+    bool desc, index_mode, wide;
+    uint32_t passes;
+    uint64_t num_segments, chunks, max_large, max_tiles, tile;
+    uint32_t lds_cap;
+    size_t off_tmp_keys, off_tmp_vals, off_idx, off_vals_copy, off_ctrl, off_chunk, off_queue, off_desc;
+    size_t off_tile_seg, off_seg_gp, off_lb, total;
+};
+
+rsort_status make_seg_plan(const rsort_desc* d, uint64_t n, uint32_t offset_bytes, uint64_t num_segments,
+                           SegPlan* p) {
+    const rsort_status st = check_desc(d);
+    if (st != RSORT_SUCCESS) return st;
+    if (offset_bytes != 4 && offset_bytes != 8)
+        return fail(RSORT_ERROR_INVALID_ARGUMENT, "offset_bytes must be 4 or 8, got %u", offset_bytes);
+    if (num_segments > 0xFFFFFFFEull) return fail(RSORT_ERROR_UNSUPPORTED, "more than 2^32-2 segments");
+    if (offset_bytes == 4 && n > 0xFFFFFFFFull)
+        return fail(RSORT_ERROR_UNSUPPORTED, "32-bit offsets cannot address n = %llu keys", (unsigned long long)n);
+    memset(p, 0, sizeof(*p));
+    p->kind = d->key_kind;
+    p->kb = d->key_bytes;
+    p->vb_user = d->value_bytes;
+    p->desc = d->descending != 0;
+    p->index_mode = d->value_bytes != 0 && !is_pow2_width(d->value_bytes);
+    p->vb_core = p->index_mode ? 4u : d->value_bytes;
+    if (p->index_mode && n > 0xFFFFFFFFull)
+        return fail(RSORT_ERROR_UNSUPPORTED, "values of %u bytes need n < 2^32", d->value_bytes);
+    p->wide = n > kNarrowMaxN;
+    p->passes = p->kb;
+    p->num_segments = num_segments;
+    p->chunks = (num_segments + kSegChunk - 1) / kSegChunk;
+    const TileCfg c = tile_cfg((int)p->kb, (int)p->vb_core);
+    p->tile = (uint64_t)c.threads * c.items;
+    p->lds_cap = seg_cap((int)p->kb, (int)p->vb_core, kSegBins - 1);
+    p->max_large = num_segments ? n / ((uint64_t)p->lds_cap + 1u) : 0;
+    if (p->max_large > num_segments) p->max_large = num_segments;
+    p->max_tiles = p->max_large ? (n + p->tile - 1) / p->tile + p->max_large : 0;
+    if (p->max_tiles > 0x7FFFFFFFull) return fail(RSORT_ERROR_UNSUPPORTED, "n too large");
+
+    const size_t off_sz = p->wide ? 8 : 4;
+    size_t off = 0;
+    if (p->max_large) {
+        p->off_tmp_keys = reserve(&off, (size_t)n * p->kb);
+        p->off_tmp_vals = p->vb_core ? reserve(&off, (size_t)n * p->vb_core) : 0;
+    }
+    if (p->index_mode) {
+        p->off_idx = reserve(&off, (size_t)n * 4u);
+        p->off_vals_copy = reserve(&off, (size_t)n * p->vb_user);
+    }
+    p->off_ctrl = reserve(&off, sizeof(SegCtrl));
+    p->off_chunk = reserve(&off, (size_t)p->chunks * 8u * 4u);
+    p->off_queue = reserve(&off, (size_t)num_segments * 4u);
+    p->off_desc = reserve(&off, (size_t)p->max_large * 32u);
+    p->off_tile_seg = reserve(&off, (size_t)p->max_tiles * 4u);
+    p->off_seg_gp = reserve(&off, (size_t)p->max_large * p->passes * kRadix * off_sz);
+    p->off_lb = reserve(&off, (size_t)p->max_tiles * kRadix * off_sz);
+    p->total = off + 256;
     return RSORT_SUCCESS;
 }
 
@@ -235,6 +305,91 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
     return RSORT_SUCCESS;
 }
 
+hipError_t dispatch_seg(const SegJob& j, hipStream_t s) {
+    switch (j.kb) {
+        case 1: return launch_seg_kb1(j, s);
+        case 2: return launch_seg_kb2(j, s);
+        case 4: return launch_seg_kb4(j, s);
+        case 8: return launch_seg_kb8(j, s);
+        case 16: return launch_seg_kb16(j, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+rsort_status run_segmented(const rsort_desc* d, void* keys, void* values, uint64_t n, const void* offsets,
+                           uint32_t offset_bytes, uint64_t num_segments, void* workspace, size_t* temp_bytes,
+                           void* stream, hipEvent_t* events, int max_events, int* n_events) {
+    SegPlan p;
+    rsort_status st = make_seg_plan(d, n, offset_bytes, num_segments, &p);
+    if (st != RSORT_SUCCESS) return st;
+    if (!temp_bytes) return fail(RSORT_ERROR_INVALID_ARGUMENT, "temp_bytes is NULL");
+    if (!workspace) {
+        *temp_bytes = p.total;
+        return RSORT_SUCCESS;
+    }
+    if (*temp_bytes < p.total)
+        return fail(RSORT_ERROR_WORKSPACE_TOO_SMALL, "workspace of %zu bytes, %zu required", *temp_bytes, p.total);
+    if (n <= 1 || num_segments == 0) return RSORT_SUCCESS;
+    if (!keys) return fail(RSORT_ERROR_INVALID_ARGUMENT, "keys is NULL");
+    if (!offsets) return fail(RSORT_ERROR_INVALID_ARGUMENT, "offsets is NULL");
+    if (p.vb_user && !values) return fail(RSORT_ERROR_INVALID_ARGUMENT, "values is NULL but value_bytes = %u", p.vb_user);
+    if ((uintptr_t)keys % p.kb) return fail(RSORT_ERROR_INVALID_ARGUMENT, "keys not aligned to %u bytes", p.kb);
+    if (p.vb_user && is_pow2_width(p.vb_user) && (uintptr_t)values % p.vb_user)
+        return fail(RSORT_ERROR_INVALID_ARGUMENT, "values not aligned to %u bytes", p.vb_user);
+    if ((uintptr_t)offsets % offset_bytes)
+        return fail(RSORT_ERROR_INVALID_ARGUMENT, "offsets not aligned to %u bytes", offset_bytes);
+    if ((uintptr_t)workspace % 16) return fail(RSORT_ERROR_INVALID_ARGUMENT, "workspace not 16-byte aligned");
+
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    const int cus = device_cus(dev);
+    hipStream_t s = (hipStream_t)stream;
+
+    uint8_t* base = (uint8_t*)(((uintptr_t)workspace + 255u) & ~(uintptr_t)255u);
+    SegJob j{};
+    j.kb = p.kb;
+    j.vb = p.vb_core;
+    j.wide = p.wide;
+    j.tw = make_tw_masks(p.kind, p.kb, p.desc);
+    j.keys = keys;
+    j.vals = p.index_mode ? (void*)(base + p.off_idx) : values;
+    j.tmp_keys = p.max_large ? base + p.off_tmp_keys : nullptr;
+    j.tmp_vals = (p.max_large && p.vb_core) ? base + p.off_tmp_vals : nullptr;
+    j.offsets = offsets;
+    j.off64 = offset_bytes == 8 ? 1u : 0u;
+    j.num_segments = num_segments;
+    j.n = n;
+    j.ctrl = (SegCtrl*)(base + p.off_ctrl);
+    j.chunk = (uint32_t*)(base + p.off_chunk);
+    j.queue = (uint32_t*)(base + p.off_queue);
+    j.desc = base + p.off_desc;
+    j.tile_seg = (uint32_t*)(base + p.off_tile_seg);
+    j.seg_gp = base + p.off_seg_gp;
+    j.lookback = base + p.off_lb;
+    j.chunks = (uint32_t)p.chunks;
+    j.max_large = p.max_large;
+    j.max_tiles = p.max_tiles;
+    j.num_cus = cus;
+    j.events = events;
+    j.max_events = max_events;
+    j.n_events = n_events;
+
+    if (p.index_mode) {
+        e = launch_iota((uint32_t*)j.vals, n, cus, s);
+        if (e != hipSuccess) return hip_fail(e, "index fill");
+    }
+    e = dispatch_seg(j, s);
+    if (e != hipSuccess) return hip_fail(e, "segmented sort launch");
+    if (p.index_mode) {
+        void* copy = base + p.off_vals_copy;
+        e = launch_gather(copy, values, (const uint32_t*)j.vals, n, p.vb_user, cus, s);
+        if (e == hipSuccess) e = launch_copy(values, copy, n * p.vb_user, cus, s);
+        if (e != hipSuccess) return hip_fail(e, "value gather");
+    }
+    return RSORT_SUCCESS;
+}
+
 }  // namespace
 
 hipError_t launch_iota(uint32_t* p, uint64_t n, int num_cus, hipStream_t s) {
@@ -310,6 +465,29 @@ rsort_status rsort_sort_profiled(const rsort_desc* desc, void* keys, void* value
     }
     for (int i = 0; i < made; ++i) (void)hipEventDestroy(ev[i]);
     return st;
+}
+
+rsort_status rsort_segmented_sort(const rsort_desc* desc, void* keys, void* values, uint64_t n, const void* offsets,
+                                  uint32_t offset_bytes, uint64_t num_segments, void* workspace, size_t* temp_bytes,
+                                  void* stream) {
+    return rsa::run_segmented(desc, keys, values, n, offsets, offset_bytes, num_segments, workspace, temp_bytes,
+                              stream, nullptr, 0, nullptr);
+}
+
+rsort_status rsort_segmented_plan(const rsort_desc* desc, uint64_t n, uint32_t offset_bytes, uint64_t num_segments,
+                                  rsort_segmented_plan_info* out) {
+    if (!out) return rsa::fail(RSORT_ERROR_INVALID_ARGUMENT, "plan output is NULL");
+    rsa::SegPlan p;
+    rsort_status st = rsa::make_seg_plan(desc, n, offset_bytes, num_segments, &p);
+    if (st != RSORT_SUCCESS) return st;
+    for (int b = 0; b < RSORT_SEGMENT_BINS; ++b) out->bin_capacity[b] = rsa::seg_cap((int)p.kb, (int)p.vb_core, b);
+    out->tile = (uint32_t)p.tile;
+    out->max_long_segments = p.max_large;
+    out->max_tiles = p.max_tiles;
+    out->wide = p.wide;
+    out->index_mode = p.index_mode;
+    out->workspace_bytes = p.total;
+    return RSORT_SUCCESS;
 }
 
 rsort_status rsort_plan(const rsort_desc* desc, uint64_t n, rsort_plan_info* out) {

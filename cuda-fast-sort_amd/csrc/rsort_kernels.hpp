@@ -214,31 +214,37 @@ __device__ __forceinline__ T wave_incl_scan(T x) {
 // workgroup adds the wall-clock ticks (100 MHz) of each phase to a slot of
 // the debug buffer; 64 copies of the slots spread the atomics.
 #ifdef RSA_PHASE_TIMING
-#define RSA_TIMER_START(dbgbase)                                                              \
-    unsigned long long* rsa_dbg = (dbgbase) ? (dbgbase) + (blockIdx.x & 63u) * 32u : nullptr; \
-    uint64_t rsa_t = wall_clock64();                                                          \
-    const uint64_t rsa_t0 = rsa_t;
-#define RSA_MARK(slot)                                                      \
-    do {                                                                    \
-        if (threadIdx.x == 0 && rsa_dbg) {                                  \
-            const uint64_t n_ = wall_clock64();                             \
-            atomicAdd(rsa_dbg + (slot), (unsigned long long)(n_ - rsa_t));  \
-            rsa_t = n_;                                                     \
-        }                                                                   \
+struct PhaseTimer {
+    unsigned long long* dbg;
+    uint64_t t, t0;
+};
+#define RSA_TIMER_START(dbgbase)                                                                      \
+    PhaseTimer rsa_pt{(dbgbase) ? (dbgbase) + (blockIdx.x & 63u) * 32u : nullptr, (uint64_t)wall_clock64(), 0}; \
+    rsa_pt.t0 = rsa_pt.t;
+#define RSA_MARK(slot)                                                           \
+    do {                                                                         \
+        if (threadIdx.x == 0 && rsa_pt.dbg) {                                    \
+            const uint64_t n_ = wall_clock64();                                  \
+            atomicAdd(rsa_pt.dbg + (slot), (unsigned long long)(n_ - rsa_pt.t)); \
+            rsa_pt.t = n_;                                                       \
+        }                                                                        \
     } while (0)
-#define RSA_COUNT(slot, v)                                                        \
-    do {                                                                          \
-        if (rsa_dbg) atomicAdd(rsa_dbg + (slot), (unsigned long long)(v));        \
+#define RSA_COUNT(slot, v)                                                       \
+    do {                                                                         \
+        if (rsa_pt.dbg) atomicAdd(rsa_pt.dbg + (slot), (unsigned long long)(v)); \
     } while (0)
-#define RSA_TOTAL(slot)                                                                   \
-    do {                                                                                  \
-        if (threadIdx.x == 0 && rsa_dbg) {                                                \
-            atomicAdd(rsa_dbg + (slot), (unsigned long long)(wall_clock64() - rsa_t0));   \
-            atomicAdd(rsa_dbg + (slot) + 1, 1ull);                                        \
-        }                                                                                 \
+#define RSA_TOTAL(slot)                                                                          \
+    do {                                                                                         \
+        if (threadIdx.x == 0 && rsa_pt.dbg) {                                                    \
+            atomicAdd(rsa_pt.dbg + (slot), (unsigned long long)(wall_clock64() - rsa_pt.t0));    \
+            atomicAdd(rsa_pt.dbg + (slot) + 1, 1ull);                                            \
+        }                                                                                        \
     } while (0)
 #else
-#define RSA_TIMER_START(dbgbase)
+struct PhaseTimer {};
+#define RSA_TIMER_START(dbgbase) \
+    PhaseTimer rsa_pt{};         \
+    (void)rsa_pt;
 #define RSA_MARK(slot) \
     do {               \
     } while (0)
@@ -257,6 +263,51 @@ __device__ __forceinline__ T lds_ld(T* p) {
 template <typename T>
 __device__ __forceinline__ void lds_st(T* p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+// Stable rank of each of a wave's K items among the wave's items of the same
+// digit that precede it (item order: (i, lane), i.e. item i of lane l comes
+// after items 0..i-1 of every lane and after item i of lanes 0..l-1).
+//   M  - the wave's 256 peer-mask words, zero on entry, zero on exit
+//   C  - the wave's 256 digit counters, zero on entry, the wave's digit
+//        totals on exit
+//   dr - rank << 8 | digit per item
+// A wave whose item i shares one digit takes the fast path (rank = lane);
+// otherwise the lanes of a digit find each other by or-ing their lane bit
+// into the digit's mask word, and the lowest such lane advances the counter.
+// The LDS executes one wave's operations in order, so the sequence needs only
+// compiler fences (reference: ranker.cuh:62-170 match_early_counts, which
+// builds the peer mask from 8 ballots instead).
+template <int KB, int K>
+__device__ __forceinline__ void rank_wave(const typename KeyTraits<KB>::Reg (&key)[K], uint32_t sh, uint32_t wsel,
+                                          uint64_t* M, uint32_t* C, uint32_t lane, uint32_t (&dr)[K]) {
+    const uint64_t lanebit = 1ull << lane;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        const uint32_t d = key_digit(key[i], sh, wsel);
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        uint32_t rank;
+        if (__all(d == d0)) {
+            const uint32_t c = lds_ld(C + d0);
+            rank = c + lane;
+            RSA_COMPILER_FENCE();
+            if (lane == 0) lds_st(C + d0, c + 64u);
+        } else {
+            __hip_atomic_fetch_or(M + d, lanebit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            RSA_COMPILER_FENCE();
+            const uint64_t m = lds_ld(M + d);
+            RSA_COMPILER_FENCE();
+            lds_st(M + d, (uint64_t)0);
+            RSA_COMPILER_FENCE();
+            const uint32_t c = lds_ld(C + d);
+            const uint32_t below = mask_below(m);
+            rank = c + below;
+            RSA_COMPILER_FENCE();
+            if (below == 0u) lds_st(C + d, c + (uint32_t)__builtin_popcountll(m));
+        }
+        RSA_COMPILER_FENCE();
+        dr[i] = (rank << 8) | d;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -488,22 +539,31 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     constexpr uint64_t STEP = (uint64_t)UNROLL * WAVES;
     uint64_t g = g0 + w;
     if (g + (UNROLL - 1) * WAVES < g1) {
-        uint4 cur[UNROLL], nxt[UNROLL];
+        // two register batches alternate: one is counted while the other loads
+        uint4 qa[UNROLL], qb[UNROLL];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) cur[u] = base[(g + (uint64_t)u * WAVES) * 64 + lane];
+        for (int u = 0; u < UNROLL; ++u) qa[u] = base[(g + (uint64_t)u * WAVES) * 64 + lane];
         while (true) {
-            const uint64_t gn = g + STEP;
-            const bool more = gn + (UNROLL - 1) * WAVES < g1;
+            uint64_t gn = g + STEP;
+            bool more = gn + (UNROLL - 1) * WAVES < g1;
             if (more) {
 #pragma unroll
-                for (int u = 0; u < UNROLL; ++u) nxt[u] = base[(gn + (uint64_t)u * WAVES) * 64 + lane];
+                for (int u = 0; u < UNROLL; ++u) qb[u] = base[(gn + (uint64_t)u * WAVES) * 64 + lane];
             }
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) count_group(cur[u]);
+            for (int u = 0; u < UNROLL; ++u) count_group(qa[u]);
             g = gn;
             if (!more) break;
+            gn = g + STEP;
+            more = gn + (UNROLL - 1) * WAVES < g1;
+            if (more) {
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) cur[u] = nxt[u];
+                for (int u = 0; u < UNROLL; ++u) qa[u] = base[(gn + (uint64_t)u * WAVES) * 64 + lane];
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) count_group(qb[u]);
+            g = gn;
+            if (!more) break;
         }
     }
     for (; g < g1; g += WAVES) count_group(base[g * 64 + lane]);
@@ -625,6 +685,76 @@ __device__ __forceinline__ void lb_store(W* p, W v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// LDS of one onesweep tile.  The staging buffer `big` holds the per-wave
+// peer-mask tables while ranking, then the tile's keys, then its values.
+template <int KB, int VB, bool WIDE, int THREADS, int K>
+struct OnesweepLds {
+    using KM = typename KeyTraits<KB>::Mem;
+    using VM = typename ValTraits<VB>::Mem;
+    using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
+    static constexpr int WAVES = THREADS / 64;
+    static constexpr uint32_t TILE = (uint32_t)THREADS * K;
+    static constexpr size_t KSTAGE = (size_t)TILE * sizeof(KM);
+    static constexpr size_t VSTAGE = VB ? (size_t)TILE * sizeof(VM) : 0;
+    static constexpr size_t MATCH = (size_t)WAVES * 256 * sizeof(uint64_t);
+    static constexpr size_t BIG =
+        KSTAGE > VSTAGE ? (KSTAGE > MATCH ? KSTAGE : MATCH) : (VSTAGE > MATCH ? VSTAGE : MATCH);
+    static_assert(THREADS % 64 == 0 && THREADS >= 256, "workgroup must hold 256 digit owners");
+    static_assert(TILE <= 65535u * 64u, "tile too large");
+
+    __attribute__((aligned(16))) uint8_t big[BIG];
+    __attribute__((aligned(16))) uint32_t cnt[WAVES * 256];
+    Off base[256];
+    uint32_t wsum[4];
+    uint32_t tile;
+#ifdef RSA_PHASE_TIMING
+    uint32_t dbg[2];
+#endif
+};
+
+// Where one executed pass reads and writes, and what it applies.
+template <int KB, int VB>
+struct PassIo {
+    const typename KeyTraits<KB>::Mem* __restrict__ kin;
+    typename KeyTraits<KB>::Mem* __restrict__ kout;
+    const typename ValTraits<VB>::Mem* __restrict__ vin;
+    typename ValTraits<VB>::Mem* __restrict__ vout;
+    uint32_t sh, wsel;  // digit position (key_digit)
+    uint32_t epoch;     // lookback epoch of the pass
+    bool first, last;   // apply twiddle-in on load / twiddle-out on store
+};
+
+// One tile of a pass: its lookback row, whether it heads its lookback chain
+// (publishes an inclusive prefix at once), its first key, and the end of the
+// key range it belongs to (the tile holds keys tbase .. min(tbase+TILE, kend)).
+struct TileGeom {
+    uint32_t lb_tile;
+    bool head;
+    uint64_t tbase;
+    uint64_t kend;
+};
+
+// Take a tile ticket and clear the per-wave rank tables.  Returns the ticket
+// (every thread); the tables are ready for onesweep_tile.
+template <int KB, int VB, bool WIDE, int THREADS, int K>
+__device__ __forceinline__ uint32_t onesweep_begin(OnesweepLds<KB, VB, WIDE, THREADS, K>& L, uint32_t* ticket) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t w = tid >> 6;
+    if (tid == 0) L.tile = atomicAdd(ticket, 1u);
+    {
+        // each wave clears its own match table and digit counters
+        uint4* m = (uint4*)(L.big + (size_t)w * 256 * sizeof(uint64_t));
+        m[lane] = make_uint4(0u, 0u, 0u, 0u);
+        m[lane + 64] = make_uint4(0u, 0u, 0u, 0u);
+        ((uint4*)(L.cnt + w * 256))[lane] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    __syncthreads();
+    return L.tile;
+}
+
+// One LSD pass over the whole array: one workgroup per tile, tiles taken in
+// ticket order so every tile's predecessors are already running.
 template <int KB, int VB, bool WIDE, int THREADS, int K>
 __global__ __launch_bounds__(THREADS) void k_onesweep(PassParams a) {
     using KM = typename KeyTraits<KB>::Mem;
@@ -633,237 +763,39 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(PassParams a) {
     using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
     constexpr int WAVES = THREADS / 64;
     constexpr uint32_t TILE = (uint32_t)THREADS * K;
-    constexpr size_t KSTAGE = (size_t)TILE * sizeof(KM);
-    constexpr size_t VSTAGE = VB ? (size_t)TILE * sizeof(VM) : 0;
-    constexpr size_t MATCH = (size_t)WAVES * 256 * sizeof(uint64_t);
-    constexpr size_t BIG = KSTAGE > VSTAGE ? (KSTAGE > MATCH ? KSTAGE : MATCH) : (VSTAGE > MATCH ? VSTAGE : MATCH);
-    static_assert(THREADS % 64 == 0 && THREADS >= 256, "workgroup must hold 256 digit owners");
-    static_assert(TILE <= 65535u * 64u, "tile too large");
-
-    __shared__ __attribute__((aligned(16))) uint8_t s_big[BIG];
-    __shared__ __attribute__((aligned(16))) uint32_t s_cnt[WAVES * 256];
-    __shared__ Off s_base[256];
-    __shared__ uint32_t s_wsum[4];
-    __shared__ uint32_t s_tile;
-#ifdef RSA_PHASE_TIMING
-    __shared__ uint32_t s_dbg[2];
-#endif
+    __shared__ OnesweepLds<KB, VB, WIDE, THREADS, K> L;
 
     const uint32_t passes = a.passes;
     const uint32_t pmask = passes >= 32 ? ~0u : ((1u << passes) - 1u);
     const uint32_t active = ~a.ctrl->skip_mask & pmask;
     if (!((active >> a.pass) & 1u)) return;  // uniform digit: pass is the identity
     const uint32_t e = __builtin_popcount(active & ((1u << a.pass) - 1u));
-    const uint32_t epoch = e & 1u;
-    const bool first = (e == 0u) && a.tw.active;
-    const bool last = ((active >> (a.pass + 1u)) == 0u) && a.tw.active;
-    const KM* __restrict__ kin = (const KM*)((e & 1u) ? a.keys_b : a.keys_a);
-    KM* __restrict__ kout = (KM*)((e & 1u) ? a.keys_a : a.keys_b);
-    const VM* __restrict__ vin = (const VM*)((e & 1u) ? a.vals_b : a.vals_a);
-    VM* __restrict__ vout = (VM*)((e & 1u) ? a.vals_a : a.vals_b);
+    PassIo<KB, VB> io;
+    io.kin = (const KM*)((e & 1u) ? a.keys_b : a.keys_a);
+    io.kout = (KM*)((e & 1u) ? a.keys_a : a.keys_b);
+    io.vin = (const VM*)((e & 1u) ? a.vals_b : a.vals_a);
+    io.vout = (VM*)((e & 1u) ? a.vals_a : a.vals_b);
+    io.sh = 8u * (a.pass & 3u);
+    io.wsel = a.pass >> 2;
+    io.epoch = e & 1u;
+    io.first = (e == 0u) && a.tw.active;
+    io.last = ((active >> (a.pass + 1u)) == 0u) && a.tw.active;
 
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & 63u;
-    const uint32_t w = tid >> 6;
     RSA_TIMER_START(a.dbg);
 #ifdef RSA_PHASE_TIMING
-    if (tid < 2) s_dbg[tid] = 0u;
+    if (threadIdx.x < 2) L.dbg[threadIdx.x] = 0u;
 #endif
-
-    if (tid == 0) s_tile = atomicAdd(&a.ctrl->ticket[a.pass], 1u);
-    {
-        // each wave clears its own match table and digit counters
-        uint4* m = (uint4*)(s_big + (size_t)w * 256 * sizeof(uint64_t));
-        m[lane] = make_uint4(0u, 0u, 0u, 0u);
-        m[lane + 64] = make_uint4(0u, 0u, 0u, 0u);
-        ((uint4*)(s_cnt + w * 256))[lane] = make_uint4(0u, 0u, 0u, 0u);
-    }
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint64_t tbase = (uint64_t)tile * TILE;
-    const bool full = tbase + TILE <= a.n;
-    const uint32_t valid = full ? TILE : (uint32_t)(a.n - tbase);
-    RSA_MARK(0);
-
-    // ---- load (wave-striped: wave w owns TILE/WAVES consecutive keys) ----
-    KR key[K];
-    const uint64_t wb = tbase + (uint64_t)w * 64u * K + lane;
-    if (full) {
-#pragma unroll
-        for (int i = 0; i < K; ++i) key[i] = key_to_reg<KB>(kin[wb + (uint64_t)i * 64u]);
-        if (first) {
-#pragma unroll
-            for (int i = 0; i < K; ++i) key[i] = tw_in<KB>(key[i], a.tw);
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < K; ++i) {
-            const uint64_t idx = wb + (uint64_t)i * 64u;
-            if (idx < a.n) {
-                KR x = key_to_reg<KB>(kin[idx]);
-                if (first) x = tw_in<KB>(x, a.tw);
-                key[i] = x;
-            } else {
-                key[i] = key_filler<KB>();  // sorts last in the tile
-            }
-        }
-    }
-
-#ifdef RSA_PHASE_TIMING
-    __builtin_amdgcn_s_waitcnt(0);  // attribute the load latency to its own phase
-#endif
-    RSA_MARK(1);
-
-    // ---- rank: stable rank of each item among same-digit items of its wave ----
-    const uint32_t sh = 8u * (a.pass & 3u);
-    const uint32_t wsel = a.pass >> 2;
-    uint64_t* M = (uint64_t*)s_big + (size_t)w * 256;
-    uint32_t* C = s_cnt + w * 256;
-    const uint64_t lanebit = 1ull << lane;
-    uint32_t dr[K];  // rank << 8 | digit
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-        const uint32_t d = key_digit(key[i], sh, wsel);
-        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-        uint32_t rank;
-        if (__all(d == d0)) {
-            const uint32_t c = lds_ld(C + d0);
-            rank = c + lane;
-            RSA_COMPILER_FENCE();
-            if (lane == 0) lds_st(C + d0, c + 64u);
-        } else {
-            // peer mask = lanes of this wave holding the same digit
-            __hip_atomic_fetch_or(M + d, lanebit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-            RSA_COMPILER_FENCE();
-            const uint64_t m = lds_ld(M + d);
-            RSA_COMPILER_FENCE();
-            lds_st(M + d, (uint64_t)0);
-            RSA_COMPILER_FENCE();
-            const uint32_t c = lds_ld(C + d);
-            const uint32_t below = mask_below(m);
-            rank = c + below;
-            RSA_COMPILER_FENCE();
-            if (below == 0u) lds_st(C + d, c + (uint32_t)__builtin_popcountll(m));
-        }
-        RSA_COMPILER_FENCE();
-        dr[i] = (rank << 8) | d;
-    }
-    __syncthreads();
-    RSA_MARK(2);
-
-    // ---- per digit: wave prefix, early aggregate publish, tile scan ----
+    const uint32_t tile = onesweep_begin(L, &a.ctrl->ticket[a.pass]);
+    TileGeom g;
+    g.lb_tile = tile;
+    g.head = tile == 0u;
+    g.tbase = (uint64_t)tile * TILE;
+    g.kend = a.n;
+    const Off* gp_row = (const Off*)a.gp + a.pass * 256u;
     Off* lb = (Off*)a.lookback;
-    uint32_t total = 0, incl = 0, pub = 0;
-    Off gbase = 0;
-    if (tid < 256) {
-        gbase = ((const Off*)a.gp)[a.pass * 256u + tid];
-        uint32_t s = 0;
-#pragma unroll
-        for (int ww = 0; ww < WAVES; ++ww) {
-            const uint32_t c = s_cnt[ww * 256 + tid];
-            s_cnt[ww * 256 + tid] = s;
-            s += c;
-        }
-        total = s;
-        pub = total - ((tid == 255u) ? (TILE - valid) : 0u);  // fillers are not keys
-        lb_store(lb + (size_t)tile * kRadix + tid,
-                 lb_pack<Off>(tile == 0u ? kLbPrefix : kLbAggregate, epoch, (Off)pub));
-        incl = wave_incl_scan<uint32_t>(total);
-        if (lane == 63u) s_wsum[w] = incl;
-    }
-    __syncthreads();
-    RSA_MARK(3);
-    uint32_t tds = 0;
-    if (tid < 256) {
-        uint32_t woff = 0;
-        for (uint32_t ww = 0; ww < w; ++ww) woff += s_wsum[ww];
-        tds = woff + incl - total;  // tile-local start of digit tid
-#pragma unroll
-        for (int ww = 0; ww < WAVES; ++ww) s_cnt[ww * 256 + tid] += tds;
-    }
-    __syncthreads();
-
-    // ---- stage keys at their tile rank (tile-local: needs no global prefix),
-    //      which frees the key registers for the lookback window ----
-    KM* SK = (KM*)s_big;
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-        const uint32_t d = dr[i] & 255u;
-        SK[s_cnt[w * 256 + d] + (dr[i] >> 8)] = key_to_mem<KB>(key[i]);
-    }
-    RSA_MARK(5);
-
-    // ---- decoupled lookback: global prefix of each digit over earlier tiles ----
-    if (tid < 256) {
-        Off prefix = 0;
-        if (tile > 0u) {
-            uint32_t steps = 0, spins = 0;
-            const Off* col = lb + tid;
-            prefix = lookback_walk<Off, RSA_LB_W0, RSA_LB_WMAX>(
-                (int64_t)tile, epoch, [col](int64_t t) { return lb_load(col + t * (int64_t)kRadix); },
-                []() { __builtin_amdgcn_s_sleep(1); }, steps, spins);
-            lb_store(lb + (size_t)tile * kRadix + tid, lb_pack<Off>(kLbPrefix, epoch, prefix + (Off)pub));
-#ifdef RSA_PHASE_TIMING
-            atomicAdd(&s_dbg[0], steps);
-            atomicAdd(&s_dbg[1], spins);
-#endif
-        }
-        s_base[tid] = gbase + prefix - (Off)tds;
-    }
-    // values are needed only after the key scatter: load them now so the
-    // loads overlap the barrier and the key writes
-    VM val[VB > 0 ? K : 1];
-    if constexpr (VB > 0) {
-        if (full) {
-#pragma unroll
-            for (int i = 0; i < K; ++i) val[i] = vin[wb + (uint64_t)i * 64u];
-        } else {
-#pragma unroll
-            for (int i = 0; i < K; ++i) {
-                const uint64_t idx = wb + (uint64_t)i * 64u;
-                val[i] = idx < a.n ? vin[idx] : VM{};
-            }
-        }
-    }
-    __syncthreads();
-    RSA_MARK(4);
-#ifdef RSA_PHASE_TIMING
-    if (tid == 0) {
-        RSA_COUNT(8, s_dbg[0]);
-        RSA_COUNT(9, s_dbg[1]);
-    }
-#endif
-
-    // ---- write coalesced digit runs of the staged keys ----
-    Off gi[VB > 0 ? K : 1];
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-        const uint32_t pos = (uint32_t)j * THREADS + tid;
-        if (full || pos < valid) {
-            KR k = key_to_reg<KB>(SK[pos]);
-            const Off g = s_base[key_digit(k, sh, wsel)] + (Off)pos;
-            if (last) k = tw_out<KB>(k, a.tw);
-            kout[g] = key_to_mem<KB>(k);
-            if constexpr (VB > 0) gi[j] = g;
-        }
-    }
-    RSA_MARK(6);
-    if constexpr (VB > 0) {
-        __syncthreads();
-        VM* SV = (VM*)s_big;
-#pragma unroll
-        for (int i = 0; i < K; ++i) {
-            const uint32_t d = dr[i] & 255u;
-            SV[s_cnt[w * 256 + d] + (dr[i] >> 8)] = val[i];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            const uint32_t pos = (uint32_t)j * THREADS + tid;
-            if (full || pos < valid) vout[gi[j]] = SV[pos];
-        }
-        RSA_MARK(7);
-    }
+    const TwMasks& tw = a.tw;
+    const uint32_t tile_tid = threadIdx.x;
+#include "rsort_onesweep_tile.inc"
     RSA_TOTAL(10);
 }
 

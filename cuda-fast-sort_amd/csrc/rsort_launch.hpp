@@ -40,12 +40,49 @@ struct SortJob {
     int* n_events;
 };
 
+// One segmented sort (no reference counterpart): classification, LDS bins,
+// then the onesweep passes over the long segments.
+struct SegJob {
+    uint32_t kb;          // key bytes 1,2,4,8,16
+    uint32_t vb;          // value bytes as seen by the kernels: 0,1,2,4,8,16
+    bool wide;            // 64-bit offsets and lookback words
+    TwMasks tw;
+    void* keys;
+    void* vals;
+    void* tmp_keys;       // ping-pong buffers (long segments)
+    void* tmp_vals;
+    const void* offsets;  // num_segments + 1 entries
+    uint32_t off64;       // 64-bit offsets
+    uint64_t num_segments;
+    uint64_t n;
+    SegCtrl* ctrl;
+    uint32_t* chunk;
+    uint32_t* queue;
+    void* desc;           // SegDesc[max_large]
+    uint32_t* tile_seg;
+    void* seg_gp;
+    void* lookback;
+    uint32_t chunks;      // ceil(num_segments / kSegChunk)
+    uint64_t max_large;   // bound on long segments
+    uint64_t max_tiles;   // bound on their tiles
+    int num_cus;
+    hipEvent_t* events;
+    int max_events;
+    int* n_events;
+};
+
 // One launcher per key width, each in its own translation unit.
 hipError_t launch_core_kb1(const SortJob& j, hipStream_t s);
 hipError_t launch_core_kb2(const SortJob& j, hipStream_t s);
 hipError_t launch_core_kb4(const SortJob& j, hipStream_t s);
 hipError_t launch_core_kb8(const SortJob& j, hipStream_t s);
 hipError_t launch_core_kb16(const SortJob& j, hipStream_t s);
+
+hipError_t launch_seg_kb1(const SegJob& j, hipStream_t s);
+hipError_t launch_seg_kb2(const SegJob& j, hipStream_t s);
+hipError_t launch_seg_kb4(const SegJob& j, hipStream_t s);
+hipError_t launch_seg_kb8(const SegJob& j, hipStream_t s);
+hipError_t launch_seg_kb16(const SegJob& j, hipStream_t s);
 
 // Helpers living in the host translation unit.
 hipError_t launch_iota(uint32_t* p, uint64_t n, int num_cus, hipStream_t s);
@@ -61,7 +98,8 @@ inline uint32_t grid_for(uint64_t work, uint32_t per_block, int num_cus, uint32_
     return (uint32_t)g;
 }
 
-inline void record_event(const SortJob& j, hipStream_t s) {
+template <typename Job>
+inline void record_event(const Job& j, hipStream_t s) {
     if (j.events && j.n_events && *j.n_events < j.max_events) {
         (void)hipEventRecord(j.events[*j.n_events], s);
         ++*j.n_events;

@@ -37,6 +37,19 @@ struct Ctrl {
 };
 static_assert(sizeof(Ctrl) == 256, "control header must stay 256 bytes");
 
+// Control block of the segmented sort (same 256-byte slot, cleared per sort).
+// Categories 0..4 are the LDS bins, 5 the onesweep segments.
+struct SegCtrl {
+    uint32_t ticket[kMaxPasses];  // onesweep pass tickets over the long segments' tiles
+    uint32_t bin_ticket[8];       // work tickets of the LDS bins
+    uint32_t cat_count[8];        // segments per category
+    uint32_t cat_base[8];         // first queue slot of each category
+    uint32_t large_tiles;         // onesweep tiles of all long segments
+    uint32_t overflow;            // 1: segments overlap beyond the workspace bounds
+    uint32_t reserved[22];
+};
+static_assert(sizeof(SegCtrl) == 256, "segmented control header must stay 256 bytes");
+
 // ---------------------------------------------------------------------------
 // Lookback words.  One word per (tile, digit).  Flag, epoch and value share
 // one machine word so a single atomic load observes a consistent triple.

@@ -1,0 +1,120 @@
+// Launcher of the segmented sort, instantiated once per key width (seg_kb*.hip).
+#pragma once
+
+#include "rsort_launch.hpp"
+#include "rsort_segmented.hpp"
+
+namespace rsa {
+
+// Resident workgroups of a kernel per CU (queried once per kernel; every
+// device this library runs on is gfx950).
+template <typename F>
+int blocks_per_cu(F kernel, int threads) {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, threads, 0) != hipSuccess || b < 1) b = 1;
+    return b;
+}
+
+template <int KB, int VB, int BIN>
+hipError_t launch_seg_bin(const SegParams& a, const SegJob& j, hipStream_t s) {
+    constexpr int THREADS = 64 << BIN;
+    constexpr int K = (int)seg_items(KB, VB);
+    static const int occ = blocks_per_cu(k_seg_lds<KB, VB, THREADS, K>, THREADS);
+    uint64_t grid = (uint64_t)j.num_cus * (uint64_t)occ;
+    if (grid > j.num_segments) grid = j.num_segments;
+    hipLaunchKernelGGL((k_seg_lds<KB, VB, THREADS, K>), dim3((uint32_t)grid), dim3(THREADS), 0, s, a, (uint32_t)BIN);
+    return hipGetLastError();
+}
+
+template <int KB, int VB, bool WIDE>
+hipError_t launch_seg_t(const SegJob& j, hipStream_t s) {
+    using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
+    using KM = typename KeyTraits<KB>::Mem;
+    using VM = typename ValTraits<VB>::Mem;
+    constexpr TileCfg C = tile_cfg(KB, VB);
+    constexpr uint32_t P = KB;
+    (void)sizeof(Off);
+
+    SegParams a{};
+    a.keys = j.keys;
+    a.vals = j.vals;
+    a.tmp_keys = j.tmp_keys;
+    a.tmp_vals = j.tmp_vals;
+    a.offsets = j.offsets;
+    a.ctrl = j.ctrl;
+    a.chunk = j.chunk;
+    a.queue = j.queue;
+    a.desc = (SegDesc*)j.desc;
+    a.tile_seg = j.tile_seg;
+    a.seg_gp = j.seg_gp;
+    a.lookback = j.lookback;
+    a.num_segments = j.num_segments;
+    a.n = j.n;
+    a.max_large = j.max_large;
+    a.max_tiles = j.max_tiles;
+    a.off64 = j.off64;
+    a.passes = P;
+    a.tile = C.threads * C.items;
+    for (int b = 0; b < kSegBins; ++b) a.cap[b] = seg_cap(KB, VB, b);
+    a.tw = j.tw;
+
+    // 1. classification and queues
+    record_event(j, s);
+    hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(64), 0, s, (uint32_t*)j.ctrl, (uint32_t)(sizeof(SegCtrl) / 4u));
+    hipLaunchKernelGGL(k_seg_count, dim3(j.chunks), dim3(kSegChunk), 0, s, a);
+    hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(512), 0, s, a, j.chunks);
+    hipLaunchKernelGGL(k_seg_fill, dim3(j.chunks), dim3(kSegChunk), 0, s, a);
+    record_event(j, s);
+
+    // 2. LDS bins, longest segments first
+    hipError_t e = launch_seg_bin<KB, VB, 4>(a, j, s);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 3>(a, j, s);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 2>(a, j, s);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 1>(a, j, s);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 0>(a, j, s);
+    if (e != hipSuccess) return e;
+    record_event(j, s);
+
+    // 3. long segments: tile table, digit offsets, onesweep passes
+    if (j.max_large == 0) return hipGetLastError();
+    const uint32_t gl = grid_for(j.max_large, 1, j.num_cus, 4);
+    hipLaunchKernelGGL((k_seg_large_init<WIDE>), dim3(gl), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_seg_hist<KB, WIDE>), dim3(grid_for(j.max_tiles, 1, j.num_cus, 2)), dim3(512), 0, s, a);
+    hipLaunchKernelGGL((k_seg_gp<WIDE>), dim3(gl), dim3(256), 0, s, a);
+    record_event(j, s);
+    static const int occ = blocks_per_cu(k_seg_onesweep<KB, VB, WIDE, C.threads, C.items>, C.threads);
+    const uint32_t go = grid_for(j.max_tiles, 1, j.num_cus, (uint32_t)occ);
+    for (uint32_t p = 0; p < P; ++p) {
+        a.pass = p;
+        hipLaunchKernelGGL((k_seg_onesweep<KB, VB, WIDE, C.threads, C.items>), dim3(go), dim3(C.threads), 0, s, a);
+        record_event(j, s);
+    }
+    if (P & 1u) {
+        const dim3 g(8, grid_for(j.max_large, 1, j.num_cus, 4));
+        hipLaunchKernelGGL((k_seg_copy_back<KM>), g, dim3(256), 0, s, a, (KM*)j.keys, (const KM*)j.tmp_keys);
+        if constexpr (VB > 0)
+            hipLaunchKernelGGL((k_seg_copy_back<VM>), g, dim3(256), 0, s, a, (VM*)j.vals, (const VM*)j.tmp_vals);
+        record_event(j, s);
+    }
+    return hipGetLastError();
+}
+
+template <int KB, int VB>
+hipError_t launch_seg_v(const SegJob& j, hipStream_t s) {
+    return j.wide ? launch_seg_t<KB, VB, true>(j, s) : launch_seg_t<KB, VB, false>(j, s);
+}
+
+template <int KB>
+hipError_t launch_seg(const SegJob& j, hipStream_t s) {
+    switch (j.vb) {
+        case 0: return launch_seg_v<KB, 0>(j, s);
+        case 1: return launch_seg_v<KB, 1>(j, s);
+        case 2: return launch_seg_v<KB, 2>(j, s);
+        case 4: return launch_seg_v<KB, 4>(j, s);
+        case 8: return launch_seg_v<KB, 8>(j, s);
+        case 16: return launch_seg_v<KB, 16>(j, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace rsa

@@ -96,4 +96,25 @@ RSA_HD constexpr TileCfg tile_cfg(int key_bytes, int val_bytes) {
     return TileCfg{512, 6};
 }
 
+// ---------------------------------------------------------------------------
+// Segmented sort.  A segment of at most seg_cap(kb, vb, kSegBins-1) keys is
+// sorted by one workgroup entirely in LDS; bin b uses 64 << b threads holding
+// seg_items(kb, vb) keys each, and takes the segments too long for bin b-1.
+// Longer segments go through the onesweep passes with per-segment digit
+// offsets.  The item count keeps one bin's keys (or values) at <= 64 KiB of LDS.
+// ---------------------------------------------------------------------------
+constexpr int kSegBins = 5;
+constexpr uint32_t kSegCatLarge = kSegBins;  // category of the onesweep segments
+constexpr uint32_t kSegCatNone = 7;          // 0 or 1 key: nothing to do
+constexpr uint32_t kSegChunk = 1024;         // segments per classification workgroup
+
+RSA_HD constexpr uint32_t seg_items(int key_bytes, int val_bytes) {
+    const int wide = key_bytes > val_bytes ? key_bytes : val_bytes;
+    return wide <= 4 ? 16u : (wide == 8 ? 8u : 4u);
+}
+
+RSA_HD constexpr uint32_t seg_cap(int key_bytes, int val_bytes, int bin) {
+    return seg_items(key_bytes, val_bytes) * (64u << bin);
+}
+
 }  // namespace rsa

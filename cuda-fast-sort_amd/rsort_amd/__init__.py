@@ -66,6 +66,18 @@ class PlanInfo(ctypes.Structure):
     ]
 
 
+class SegPlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("bin_capacity", ctypes.c_uint32 * 5),
+        ("tile", ctypes.c_uint32),
+        ("max_long_segments", ctypes.c_uint64),
+        ("max_tiles", ctypes.c_uint64),
+        ("wide", ctypes.c_uint32),
+        ("index_mode", ctypes.c_uint32),
+        ("workspace_bytes", ctypes.c_uint64),
+    ]
+
+
 _lib = None
 
 
@@ -84,6 +96,12 @@ def lib() -> ctypes.CDLL:
         L.rsort_sort_profiled.restype = i32
         L.rsort_plan.argtypes = [ctypes.POINTER(Desc), u64, ctypes.POINTER(PlanInfo)]
         L.rsort_plan.restype = i32
+        u32 = ctypes.c_uint32
+        L.rsort_segmented_sort.argtypes = [ctypes.POINTER(Desc), vp, vp, u64, vp, u32, u64, vp,
+                                           ctypes.POINTER(ctypes.c_size_t), vp]
+        L.rsort_segmented_sort.restype = i32
+        L.rsort_segmented_plan.argtypes = [ctypes.POINTER(Desc), u64, u32, u64, ctypes.POINTER(SegPlanInfo)]
+        L.rsort_segmented_plan.restype = i32
         L.rsort_status_string.argtypes = [i32]
         L.rsort_status_string.restype = ctypes.c_char_p
         L.rsort_last_error.argtypes = []
@@ -206,3 +224,35 @@ def debug_phase_slots(reset: bool = True) -> list[int]:
     buf = (ctypes.c_ulonglong * 32)()
     k = lib().rsort_debug_phase_slots(buf, 32, 1 if reset else 0)
     return [int(buf[i]) for i in range(k)]
+
+
+# ---------------------------------------------------------------------------
+# Segmented sort (no reference counterpart: a batch of independent sorts)
+# ---------------------------------------------------------------------------
+def segmented_sort_raw(desc: Desc, keys_ptr, vals_ptr, n: int, offsets_ptr, offset_bytes: int, num_segments: int,
+                       ws_ptr, temp_bytes: int | None, stream=None) -> int:
+    """Direct call of rsort_segmented_sort; returns the workspace size (query) or temp_bytes."""
+    tb = ctypes.c_size_t(temp_bytes or 0)
+    _check(lib().rsort_segmented_sort(ctypes.byref(desc), keys_ptr, vals_ptr, n, offsets_ptr, offset_bytes,
+                                      num_segments, ws_ptr, ctypes.byref(tb), _stream_ptr(stream)))
+    return int(tb.value)
+
+
+def segmented_plan(desc: Desc, n: int, offset_bytes: int, num_segments: int) -> SegPlanInfo:
+    out = SegPlanInfo()
+    _check(lib().rsort_segmented_plan(ctypes.byref(desc), n, offset_bytes, num_segments, ctypes.byref(out)))
+    return out
+
+
+def segmented_sort(keys, offsets, temp_bytes, d_workspace, values=None, descending: bool = False, stream=None,
+                   key_type: KeyType | None = None) -> int:
+    """Sort every segment keys[offsets[i]:offsets[i+1]] (and values) in place.
+
+    offsets: int32/int64 device tensor of num_segments + 1 entries.
+    d_workspace None -> returns the workspace size (no device work).
+    """
+    kt = key_type or key_type_of(keys.dtype)
+    desc = make_desc(kt.kind, kt.nbytes, _vbytes(values), descending)
+    num_segments = max(int(offsets.numel()) - 1, 0)
+    return segmented_sort_raw(desc, _ptr(keys), _ptr(values), int(keys.shape[0]), _ptr(offsets),
+                              offsets.element_size(), num_segments, _ptr(d_workspace), temp_bytes, stream)

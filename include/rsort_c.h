@@ -104,6 +104,44 @@ typedef struct rsort_plan_info {
 rsort_status rsort_plan(const rsort_desc* desc, uint64_t n, rsort_plan_info* out);
 
 /*
+ * Segmented sort: sorts every segment [offsets[i], offsets[i+1]), i < num_segments,
+ * of keys (and values) independently, stably and in place; keys outside every
+ * segment are left alone.  No counterpart in the reference, which sorts one
+ * array per call: this is its sort applied to a batch of arrays packed back
+ * to back, in one enqueue (the same order inside each segment as rsort_sort).
+ *
+ *   offsets      device array of num_segments + 1 entries of offset_bytes (4 or 8)
+ *                bytes; indices into keys/values, non-decreasing, <= n.
+ *                Offsets are clamped to [0, n]; overlapping segments give an
+ *                unspecified order (never an out-of-bounds access).
+ *   n            extent of keys / values in elements
+ *   workspace    as rsort_sort: NULL -> size query into *temp_bytes
+ *
+ * A segment of at most bin_capacity[4] keys (rsort_segmented_plan; 16384 for
+ * 4-byte keys and values) is sorted by one workgroup in LDS, longer ones by
+ * the onesweep passes.  Stream-ordered, no host synchronisation, so the whole
+ * call can be captured into a hipGraph.
+ */
+rsort_status rsort_segmented_sort(const rsort_desc* desc, void* keys, void* values, uint64_t n, const void* offsets,
+                                  uint32_t offset_bytes, uint64_t num_segments, void* workspace, size_t* temp_bytes,
+                                  void* stream);
+
+#define RSORT_SEGMENT_BINS 5
+
+typedef struct rsort_segmented_plan_info {
+    uint32_t bin_capacity[RSORT_SEGMENT_BINS]; /* keys per LDS workgroup of 64<<b threads */
+    uint32_t tile;                /* onesweep tile of the long segments */
+    uint64_t max_long_segments;   /* workspace bound on segments longer than bin_capacity[4] */
+    uint64_t max_tiles;
+    uint32_t wide;
+    uint32_t index_mode;
+    uint64_t workspace_bytes;
+} rsort_segmented_plan_info;
+
+rsort_status rsort_segmented_plan(const rsort_desc* desc, uint64_t n, uint32_t offset_bytes, uint64_t num_segments,
+                                  rsort_segmented_plan_info* out);
+
+/*
  * Debug builds compiled with -DRSA_PHASE_TIMING accumulate per-phase wall-clock
  * ticks (100 MHz) of the kernels; this copies the summed slots to `out` and
  * returns how many were written (0 in normal builds).  Slots of the pass

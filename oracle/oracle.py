@@ -36,6 +36,7 @@ def lib() -> ctypes.CDLL:
         L.oracle_twiddle_out.argtypes = [i32, i32, i32, vp, vp, u64]
         L.oracle_sort.argtypes = [i32, i32, i32, i32, vp, vp, u64]
         L.oracle_sort_permutation.argtypes = [i32, i32, i32, vp, u64, vp]
+        L.oracle_segmented_sort.argtypes = [i32, i32, i32, i32, vp, vp, u64, vp, u64]
         L.oracle_lsd_pass.argtypes = [i32, i32, i32, vp, vp, vp, vp, u64]
         L.oracle_digit_counts.argtypes = [i32, i32, i32, vp, u64, vp]
         L.oracle_mix32.argtypes = [u32]
@@ -98,6 +99,24 @@ def sort(keys: np.ndarray, values: np.ndarray | None, kind: int, kb: int, desc: 
     else:
         vb, vbytes = 0, None
     assert lib().oracle_sort(kind, kb, vb, int(desc), _p(kbytes), _p(vbytes), n) == 0
+    return k, v
+
+
+def segmented_sort(keys: np.ndarray, values: np.ndarray | None, offsets, kind: int, kb: int, desc: bool,
+                   vb: int | None = None):
+    """Expected result of rsort_segmented_sort: each clamped segment sorted as sort() does."""
+    k = np.array(keys, copy=True)
+    v = None if values is None else np.array(values, copy=True)
+    kbytes = _as_bytes(k, kb)
+    n = kbytes.size // kb
+    if v is not None:
+        vb = vb if vb is not None else v.nbytes // max(n, 1)
+        vbytes = _as_bytes(v, vb) if n else v.view(np.uint8).reshape(-1)
+    else:
+        vb, vbytes = 0, None
+    off = np.ascontiguousarray(np.asarray(offsets, dtype=np.uint64))
+    s = max(off.size - 1, 0)
+    assert lib().oracle_segmented_sort(kind, kb, vb, int(desc), _p(kbytes), _p(vbytes), n, _p(off), s) == 0
     return k, v
 
 

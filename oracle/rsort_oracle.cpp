@@ -132,6 +132,23 @@ int oracle_sort(int kind, int kb, int vb, int desc, void* keys, void* vals, uint
     return 0;
 }
 
+// Segmented sort (no reference counterpart): every segment, clamped to
+// [min(off[i], n), min(max(off[i+1], begin), n)), sorted as oracle_sort sorts
+// a whole array; keys outside every segment stay where they are.
+int oracle_segmented_sort(int kind, int kb, int vb, int desc, void* keys, void* vals, uint64_t n,
+                          const uint64_t* offsets, uint64_t num_segments) {
+    if (!valid_desc(kind, kb) || vb < 0) return -1;
+    for (uint64_t s = 0; s < num_segments; ++s) {
+        uint64_t b = offsets[s] < n ? offsets[s] : n;
+        uint64_t e = offsets[s + 1] < b ? b : (offsets[s + 1] < n ? offsets[s + 1] : n);
+        if (e - b < 2) continue;
+        const int rc = oracle_sort(kind, kb, vb, desc, (uint8_t*)keys + b * kb, vals ? (uint8_t*)vals + b * vb : nullptr,
+                                   e - b);
+        if (rc != 0) return rc;
+    }
+    return 0;
+}
+
 // Stable permutation only: out[i] = original index of the i-th sorted key.
 int oracle_sort_permutation(int kind, int kb, int desc, const void* keys, uint64_t n, uint64_t* out) {
     if (!valid_desc(kind, kb)) return -1;

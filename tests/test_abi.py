@@ -111,3 +111,54 @@ def test_lookback_protocol_simulation(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert ": 0 errors" in r.stdout
+
+
+# ---------------------------------------------------------------------------
+# Segmented sort: plan, size query, argument checks (no device work)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("kb,vb,cap", [(4, 4, 16384), (4, 0, 16384), (8, 8, 8192), (16, 8, 4096), (1, 0, 16384),
+                                       (2, 16, 4096), (4, 12, 16384)])
+def test_segmented_plan_bins(rs, kb, vb, cap):
+    d = rs.make_desc(0, kb, vb)
+    p = rs.segmented_plan(d, 1 << 22, 4, 1000)
+    caps = list(p.bin_capacity)
+    assert caps[-1] == cap and caps == [cap >> (4 - b) for b in range(5)]
+    # at most n / (cap + 1) segments can be too long for the LDS bins
+    assert p.max_long_segments == min(1000, (1 << 22) // (cap + 1))
+    assert p.max_tiles == ((1 << 22) + p.tile - 1) // p.tile + p.max_long_segments
+    assert p.index_mode == (1 if vb == 12 else 0)
+    q = rs.segmented_sort_raw(d, None, None, 1 << 22, None, 4, 1000, None, 0)
+    assert q == p.workspace_bytes
+
+
+def test_segmented_plan_small_n_needs_no_long_path(rs):
+    d = rs.make_desc(0, 4, 4)
+    p = rs.segmented_plan(d, 16384, 4, 10)
+    assert p.max_long_segments == 0 and p.max_tiles == 0
+    big = rs.segmented_plan(d, 16385, 4, 10)
+    assert big.max_long_segments == 1
+
+
+@pytest.mark.parametrize("args,status", [
+    (dict(offset_bytes=2), 1),
+    (dict(offset_bytes=4, n=1 << 33), 2),
+    (dict(num_segments=1 << 32), 2),
+])
+def test_segmented_invalid_arguments(rs, args, status):
+    a = dict(n=1000, offset_bytes=4, num_segments=10)
+    a.update(args)
+    with pytest.raises(rs.RsortError) as e:
+        rs.segmented_sort_raw(rs.make_desc(0, 4, 0), None, None, a["n"], None, a["offset_bytes"], a["num_segments"],
+                              None, 0)
+    assert e.value.status == status
+
+
+def test_segmented_undersized_workspace_and_noops(rs):
+    d = rs.make_desc(0, 4, 4)
+    need = rs.segmented_sort_raw(d, None, None, 5000, None, 8, 7, None, 0)
+    with pytest.raises(rs.RsortError) as e:
+        rs.segmented_sort_raw(d, 0x1000, 0x2000, 5000, 0x3000, 8, 7, 0x100000, need - 1)
+    assert e.value.status == 3
+    # no segments / n <= 1: nothing is launched, no pointer is touched
+    rs.segmented_sort_raw(d, 0x1000, 0x2000, 5000, 0x3000, 8, 0, 0x100000, need)
+    rs.segmented_sort_raw(d, 0x1000, 0x2000, 1, 0x3000, 8, 7, 0x100000, need)

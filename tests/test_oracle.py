@@ -251,3 +251,67 @@ def test_generators_shapes_and_statistics(orc):
     assert s.min() >= 32 and s.max() <= (1 << 22)
     assert abs(np.median(s) - 2048) < 40
     assert abs(s.sum() / 2**28 - 1.0) < 0.03  # total ~ 2^28 keys
+
+
+# ---------------------------------------------------------------------------
+# Segmented sort oracle: each clamped segment sorted independently.
+# ---------------------------------------------------------------------------
+def _np_words(orc, keys, kind, kb, desc):
+    w = orc.twiddle_in(keys, kind, kb, desc).reshape(-1, kb)
+    return w
+
+
+@pytest.mark.parametrize("kind,kb", [(0, 4), (1, 2), (2, 8), (0, 1)])
+@pytest.mark.parametrize("desc", [False, True])
+def test_segmented_oracle_matches_numpy_per_segment(orc, kind, kb, desc):
+    """Per segment, numpy's stable argsort of the big-endian twiddled words."""
+    from helpers import random_keys
+
+    rng = np.random.default_rng(kb * 10 + kind + int(desc))
+    n = 3000
+    keys = random_keys(kind, kb, n, seed=kb + 3 * kind)
+    vals = np.arange(n, dtype=np.uint32)
+    cuts = np.sort(rng.integers(0, n, size=40))
+    offsets = np.concatenate([[5], cuts[(cuts > 5) & (cuts < n - 7)], [n - 7]]).astype(np.uint64)
+    gk, gv = orc.segmented_sort(keys, vals, offsets, kind, kb, desc, vb=4)
+    words = _np_words(orc, keys, kind, kb, desc)
+    big_endian = words[:, ::-1].copy().view(f"S{kb}").reshape(-1)  # bytes compare as unsigned
+    expect = np.arange(n, dtype=np.uint32)
+    for b, e in zip(offsets[:-1], offsets[1:]):
+        b, e = int(b), int(e)
+        if e - b >= 2:
+            expect[b:e] = b + np.argsort(big_endian[b:e], kind="stable")
+    assert np.array_equal(gv, expect)
+    assert np.array_equal(gk.reshape(-1, kb), keys.reshape(-1, kb)[expect])
+
+
+def test_segmented_oracle_single_segment_is_sort(orc):
+    from helpers import random_keys
+
+    keys = random_keys(2, 4, 5000, seed=9)
+    a, _ = orc.segmented_sort(keys, None, [0, 5000], 2, 4, True)
+    b, _ = orc.sort(keys, None, 2, 4, True)
+    assert np.array_equal(a, b)
+
+
+def test_segmented_oracle_clamps_and_leaves_gaps(orc):
+    keys = np.arange(100, 0, -1, dtype=np.uint32)
+    # [10,20) sorted, [20,20) empty, [20,30) sorted, [30,1000) clamped to
+    # [30,100), [1000,50) clamped to the empty [100,100)
+    off = [10, 20, 20, 30, 1000, 50]
+    k, _ = orc.segmented_sort(keys, None, off, 0, 4, False)
+    exp = keys.copy()
+    exp[10:20] = np.sort(keys[10:20])
+    exp[20:30] = np.sort(keys[20:30])
+    exp[30:100] = np.sort(keys[30:100])
+    assert np.array_equal(k, exp)
+
+
+def test_segmented_oracle_planted_mistake_is_caught(orc):
+    """A segment boundary shifted by one changes the result: the per-segment
+    reference above would catch an off-by-one in the oracle's clamping."""
+    keys = np.array([5, 4, 3, 2, 1, 0], dtype=np.uint32)
+    good, _ = orc.segmented_sort(keys, None, [0, 3, 6], 0, 4, False)
+    bad, _ = orc.segmented_sort(keys, None, [0, 4, 6], 0, 4, False)
+    assert good.tolist() == [3, 4, 5, 0, 1, 2]
+    assert not np.array_equal(good, bad)
