@@ -1,0 +1,548 @@
+// Serving loop: continuous batching of sort requests into segmented sorts.
+//
+// No counterpart in the reference, which sorts one caller array per call
+// (its per-call cost: graph capture + instantiate + launch, radix_kernel.cuh:
+// 583-600).  Here many small requests share one launch sequence:
+//
+//   submit()      a request = device keys (+ values) sorted in place, like
+//                 rsort_sort.  Admission: it must fit one batch slot (keys and
+//                 request count); the pending queue is bounded (BUSY when full).
+//   scheduler     a host thread closes a batch when the slot is full, when its
+//                 oldest request has waited `window_us`, or on flush, and
+//                 launches it on a free slot (one stream per slot, `depth`
+//                 slots in flight).
+//   batch         k_serve_gather copies every request into the slot's device
+//                 buffer (copy list read from host-mapped memory) and writes
+//                 the segment offsets; the segmented sort sorts all requests;
+//                 k_serve_scatter copies the results back.  The three steps
+//                 are captured once per slot into a hipGraph (fixed capacity
+//                 => fixed launch parameters) and replayed for every batch.
+//   completion    a second host thread waits for each slot's event in launch
+//                 order and stamps its requests' completion time.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/rsort_c.h"
+#include "rsort_tuning.hpp"
+
+namespace rsa {
+
+// One contiguous piece of one request (<= kServeChunk keys).
+struct ServeChunk {
+    uint64_t keys;    // device address of the piece's first key
+    uint64_t vals;    // device address of its first value (0 without values)
+    uint64_t len;     // keys in the piece
+    uint64_t offset;  // position of the piece in the batch buffer
+};
+
+struct ServeHeader {
+    uint64_t chunks;    // pieces in this batch
+    uint64_t requests;  // requests in this batch
+    uint64_t total;     // keys in this batch
+    uint64_t pad;
+};
+
+constexpr uint64_t kServeChunk = 65536;
+
+// Copy `count` elements of type E (a key or value word, or a slice of one).
+template <typename E>
+__device__ __forceinline__ void serve_copy(E* __restrict__ dst, const E* __restrict__ src, uint64_t count) {
+    for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) dst[i] = src[i];
+}
+
+// Batch copy in: pieces -> batch buffer, and the segment offsets
+// offsets[r] = first key of request r, offsets[r >= requests] = total.
+// KE: key word type; VE: value element type, vw elements per value (0: none).
+template <typename KE, typename VE>
+__global__ __launch_bounds__(256) void k_serve_gather(const ServeHeader* __restrict__ hdr,
+                                                      const ServeChunk* __restrict__ ch,
+                                                      const uint64_t* __restrict__ host_off, KE* bkeys, VE* bvals,
+                                                      uint32_t vw, uint64_t* offsets, uint32_t cap_requests) {
+    const uint64_t nch = hdr->chunks;
+    const uint64_t nreq = hdr->requests;
+    const uint64_t total = hdr->total;
+    for (uint64_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const ServeChunk p = ch[c];
+        serve_copy<KE>(bkeys + p.offset, (const KE*)p.keys, p.len);
+        if (vw) serve_copy<VE>(bvals + p.offset * vw, (const VE*)p.vals, p.len * vw);
+    }
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= cap_requests;
+         r += (uint64_t)gridDim.x * blockDim.x)
+        offsets[r] = r < nreq ? host_off[r] : total;
+}
+
+// Batch copy out: batch buffer -> each request's own buffers.
+template <typename KE, typename VE>
+__global__ __launch_bounds__(256) void k_serve_scatter(const ServeHeader* __restrict__ hdr,
+                                                       const ServeChunk* __restrict__ ch, const KE* bkeys,
+                                                       const VE* bvals, uint32_t vw) {
+    const uint64_t nch = hdr->chunks;
+    for (uint64_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const ServeChunk p = ch[c];
+        serve_copy<KE>((KE*)p.keys, bkeys + p.offset, p.len);
+        if (vw) serve_copy<VE>((VE*)p.vals, bvals + p.offset * vw, p.len * vw);
+    }
+}
+
+template <int B>
+struct Word;
+template <> struct Word<1> { using T = uint8_t; };
+template <> struct Word<2> { using T = uint16_t; };
+template <> struct Word<4> { using T = uint32_t; };
+template <> struct Word<8> { using T = uint64_t; };
+template <> struct Word<16> { using T = uint4; };
+
+struct CopyArgs {
+    const ServeHeader* hdr;
+    const ServeChunk* chunks;
+    const uint64_t* host_off;
+    void* bkeys;
+    void* bvals;
+    uint32_t vw;           // value elements per value
+    uint64_t* offsets;
+    uint32_t cap_requests;
+    uint32_t grid;
+};
+
+template <int KB, int VE>
+void launch_copy_t(const CopyArgs& c, bool in, hipStream_t s) {
+    using K = typename Word<KB>::T;
+    using V = typename Word<VE>::T;
+    if (in)
+        hipLaunchKernelGGL((k_serve_gather<K, V>), dim3(c.grid), dim3(256), 0, s, c.hdr, c.chunks, c.host_off,
+                           (K*)c.bkeys, (V*)c.bvals, c.vw, c.offsets, c.cap_requests);
+    else
+        hipLaunchKernelGGL((k_serve_scatter<K, V>), dim3(c.grid), dim3(256), 0, s, c.hdr, c.chunks, (const K*)c.bkeys,
+                           (const V*)c.bvals, c.vw);
+}
+
+template <int KB>
+void launch_copy_k(const CopyArgs& c, uint32_t ve, bool in, hipStream_t s) {
+    switch (ve) {
+        case 1: launch_copy_t<KB, 1>(c, in, s); break;
+        case 2: launch_copy_t<KB, 2>(c, in, s); break;
+        case 4: launch_copy_t<KB, 4>(c, in, s); break;
+        case 8: launch_copy_t<KB, 8>(c, in, s); break;
+        default: launch_copy_t<KB, 16>(c, in, s); break;
+    }
+}
+
+hipError_t launch_serve_copy(const CopyArgs& c, uint32_t kb, uint32_t ve, bool in, hipStream_t s) {
+    switch (kb) {
+        case 1: launch_copy_k<1>(c, ve, in, s); break;
+        case 2: launch_copy_k<2>(c, ve, in, s); break;
+        case 4: launch_copy_k<4>(c, ve, in, s); break;
+        case 8: launch_copy_k<8>(c, ve, in, s); break;
+        case 16: launch_copy_k<16>(c, ve, in, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+uint64_t us_since(Clock::time_point a, Clock::time_point b) {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(b - a).count();
+}
+
+struct Request {
+    void* keys;
+    void* vals;
+    uint64_t n;
+    Clock::time_point t_submit, t_launch, t_done;
+    uint64_t batch = 0;
+    bool done = false;
+};
+
+// One batch slot: device buffers, host-mapped copy list, stream, graph.
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    void* bkeys = nullptr;
+    void* bvals = nullptr;
+    uint64_t* offsets = nullptr;  // device, cap_requests + 1
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    uint8_t* host = nullptr;      // host-mapped: header | chunks | offsets
+    uint8_t* host_dev = nullptr;  // its device address
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> ids;    // requests of the batch in flight
+    uint64_t batch = 0;
+};
+
+}  // namespace
+
+struct Server {
+    rsort_serve_config cfg{};
+    int device = 0;
+    uint32_t kb = 0, vb = 0, ve = 0, vw = 0;
+    uint64_t max_chunks = 0;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_slot, cv_done;
+    std::deque<uint64_t> pending;           // request ids, FIFO
+    uint64_t pending_keys = 0;
+    std::deque<int> inflight;               // slot indices in launch order
+    std::vector<int> free_slots;
+    std::unordered_map<uint64_t, Request> req;
+    uint64_t next_id = 1, batches = 0, completed = 0, completed_keys = 0;
+    uint32_t launching = 0;                 // batches taken from `pending`, not yet in `inflight`
+    bool stop = false, flush = false, sched_done = false;
+    rsort_status error = RSORT_SUCCESS;
+    char error_text[256] = "";
+    std::thread scheduler, completer;
+
+    size_t header_bytes() const { return sizeof(ServeHeader); }
+    size_t chunks_bytes() const { return (size_t)max_chunks * sizeof(ServeChunk); }
+    size_t host_bytes() const { return header_bytes() + chunks_bytes() + ((size_t)cfg.batch_requests + 1) * 8u; }
+
+    hipError_t enqueue_batch(Slot& sl) {
+        CopyArgs c{};
+        c.hdr = (const ServeHeader*)sl.host_dev;
+        c.chunks = (const ServeChunk*)(sl.host_dev + header_bytes());
+        c.host_off = (const uint64_t*)(sl.host_dev + header_bytes() + chunks_bytes());
+        c.bkeys = sl.bkeys;
+        c.bvals = sl.bvals;
+        c.vw = vw;
+        c.offsets = sl.offsets;
+        c.cap_requests = cfg.batch_requests;
+        c.grid = 1024;
+        hipError_t e = launch_serve_copy(c, kb, ve, true, sl.stream);
+        if (e != hipSuccess) return e;
+        size_t tb = sl.ws_bytes;
+        const rsort_status st = rsort_segmented_sort(&cfg.desc, sl.bkeys, sl.bvals, cfg.batch_keys, sl.offsets, 8,
+                                                     cfg.batch_requests, sl.ws, &tb, sl.stream);
+        if (st != RSORT_SUCCESS) return hipErrorLaunchFailure;
+        return launch_serve_copy(c, kb, ve, false, sl.stream);
+    }
+
+    void fail_with(rsort_status st, const char* what, hipError_t e) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (error == RSORT_SUCCESS) {
+            error = st;
+            snprintf(error_text, sizeof(error_text), "%s: %s", what, hipGetErrorString(e));
+        }
+        stop = true;
+        cv_work.notify_all();
+        cv_slot.notify_all();
+        cv_done.notify_all();
+    }
+
+    void run_scheduler() {
+        (void)hipSetDevice(device);
+        std::unique_lock<std::mutex> lk(mu);
+        while (true) {
+            if (stop && (pending.empty() || error != RSORT_SUCCESS)) break;
+            if (pending.empty()) {
+                cv_work.wait(lk);
+                continue;
+            }
+            const auto oldest = req[pending.front()].t_submit;
+            const auto deadline = oldest + std::chrono::microseconds(cfg.window_us);
+            const bool full = pending_keys >= cfg.batch_keys || pending.size() >= cfg.batch_requests;
+            if (!full && !flush && !stop && Clock::now() < deadline) {
+                cv_work.wait_until(lk, deadline);
+                continue;
+            }
+            if (free_slots.empty()) {
+                cv_slot.wait(lk);
+                continue;
+            }
+            const int si = free_slots.back();
+            free_slots.pop_back();
+            Slot& sl = slots[si];
+            // admission into this batch: FIFO while the slot has room
+            sl.ids.clear();
+            uint64_t keys = 0;
+            while (!pending.empty() && sl.ids.size() < cfg.batch_requests) {
+                Request& r = req[pending.front()];
+                if (keys + r.n > cfg.batch_keys) break;
+                keys += r.n;
+                pending_keys -= r.n;
+                sl.ids.push_back(pending.front());
+                pending.pop_front();
+            }
+            if (pending.empty()) flush = false;
+            ++launching;
+            sl.batch = ++batches;
+            const auto now = Clock::now();
+            // copy list (host-mapped; the slot is idle, so nothing reads it)
+            ServeHeader* h = (ServeHeader*)sl.host;
+            ServeChunk* ch = (ServeChunk*)(sl.host + header_bytes());
+            uint64_t* off = (uint64_t*)(sl.host + header_bytes() + chunks_bytes());
+            uint64_t nch = 0, pos = 0;
+            for (size_t i = 0; i < sl.ids.size(); ++i) {
+                Request& r = req[sl.ids[i]];
+                r.t_launch = now;
+                r.batch = sl.batch;
+                off[i] = pos;
+                for (uint64_t b = 0; b < r.n; b += kServeChunk) {
+                    ServeChunk& c = ch[nch++];
+                    c.keys = (uint64_t)(uintptr_t)r.keys + b * kb;
+                    c.vals = vb ? (uint64_t)(uintptr_t)r.vals + b * vb : 0;
+                    c.len = r.n - b < kServeChunk ? r.n - b : kServeChunk;
+                    c.offset = pos + b;
+                }
+                pos += r.n;
+            }
+            h->chunks = nch;
+            h->requests = sl.ids.size();
+            h->total = pos;
+            lk.unlock();
+            hipError_t e = sl.exec ? hipGraphLaunch(sl.exec, sl.stream) : enqueue_batch(sl);
+            if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
+            lk.lock();
+            --launching;
+            if (e != hipSuccess) {
+                lk.unlock();
+                fail_with(RSORT_ERROR_DEVICE, "batch launch", e);
+                lk.lock();
+                break;
+            }
+            inflight.push_back(si);
+            cv_done.notify_all();  // wakes the completion thread
+        }
+        sched_done = true;
+        cv_done.notify_all();
+    }
+
+    void run_completer() {
+        (void)hipSetDevice(device);
+        std::unique_lock<std::mutex> lk(mu);
+        while (true) {
+            if (inflight.empty()) {
+                if (sched_done) break;
+                cv_done.wait(lk);
+                continue;
+            }
+            const int si = inflight.front();
+            Slot& sl = slots[si];
+            lk.unlock();
+            const hipError_t e = hipEventSynchronize(sl.done);
+            const auto now = Clock::now();
+            lk.lock();
+            inflight.pop_front();
+            if (e != hipSuccess) {
+                lk.unlock();
+                fail_with(RSORT_ERROR_DEVICE, "batch completion", e);
+                lk.lock();
+            }
+            for (uint64_t id : sl.ids) {
+                Request& r = req[id];
+                r.t_done = now;
+                r.done = true;
+                ++completed;
+                completed_keys += r.n;
+            }
+            sl.ids.clear();
+            free_slots.push_back(si);
+            cv_slot.notify_all();
+            cv_work.notify_all();
+            cv_done.notify_all();
+        }
+    }
+};
+
+}  // namespace rsa
+
+using rsa::Server;
+
+namespace {
+
+void destroy_slots(Server* s) {
+    for (auto& sl : s->slots) {
+        if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.bkeys) (void)hipFree(sl.bkeys);
+        if (sl.bvals) (void)hipFree(sl.bvals);
+        if (sl.offsets) (void)hipFree(sl.offsets);
+        if (sl.ws) (void)hipFree(sl.ws);
+        if (sl.host) (void)hipHostFree(sl.host);
+    }
+    s->slots.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+rsort_status rsort_serve_create(const rsort_serve_config* cfg, rsort_server** out) {
+    if (!cfg || !out) return RSORT_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (cfg->batch_keys < 2 || cfg->batch_requests < 1 || cfg->depth < 1 || cfg->depth > 8 || cfg->reserved)
+        return RSORT_ERROR_INVALID_ARGUMENT;
+    if (cfg->batch_requests > (1u << 24)) return RSORT_ERROR_UNSUPPORTED;
+    size_t need = 0;
+    rsort_status st = rsort_segmented_sort(&cfg->desc, nullptr, nullptr, cfg->batch_keys, nullptr, 8,
+                                           cfg->batch_requests, nullptr, &need, nullptr);
+    if (st != RSORT_SUCCESS) return st;
+
+    Server* s = new Server();
+    s->cfg = *cfg;
+    s->kb = cfg->desc.key_bytes;
+    s->vb = cfg->desc.value_bytes;
+    s->ve = s->vb == 0 ? 1 : (s->vb % 16 == 0 ? 16 : s->vb % 8 == 0 ? 8 : s->vb % 4 == 0 ? 4 : s->vb % 2 == 0 ? 2 : 1);
+    s->vw = s->vb / s->ve;
+    s->max_chunks = cfg->batch_requests + cfg->batch_keys / rsa::kServeChunk + 1;
+    if (hipGetDevice(&s->device) != hipSuccess) {
+        delete s;
+        return RSORT_ERROR_DEVICE;
+    }
+    s->slots.resize(cfg->depth);
+    hipError_t e = hipSuccess;
+    for (uint32_t i = 0; i < cfg->depth && e == hipSuccess; ++i) {
+        auto& sl = s->slots[i];
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipMalloc(&sl.bkeys, cfg->batch_keys * s->kb);
+        if (e == hipSuccess && s->vb) e = hipMalloc(&sl.bvals, cfg->batch_keys * s->vb);
+        if (e == hipSuccess) e = hipMalloc((void**)&sl.offsets, ((size_t)cfg->batch_requests + 1) * 8u);
+        sl.ws_bytes = need;
+        if (e == hipSuccess) e = hipMalloc(&sl.ws, need);
+        if (e == hipSuccess)
+            e = hipHostMalloc((void**)&sl.host, s->host_bytes(), hipHostMallocMapped | hipHostMallocCoherent);
+        if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&sl.host_dev, sl.host, 0);
+        if (e == hipSuccess) memset(sl.host, 0, s->host_bytes());
+    }
+    // one direct run per slot (empty batch: also initialises the launchers'
+    // one-time queries), then capture the batch into a graph
+    for (uint32_t i = 0; i < cfg->depth && e == hipSuccess; ++i) {
+        auto& sl = s->slots[i];
+        e = s->enqueue_batch(sl);
+        if (e == hipSuccess) e = hipStreamSynchronize(sl.stream);
+        if (e == hipSuccess && !(cfg->flags & RSORT_SERVE_NO_GRAPH)) {
+            hipGraph_t g = nullptr;
+            e = hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                const hipError_t e1 = s->enqueue_batch(sl);
+                e = hipStreamEndCapture(sl.stream, &g);
+                if (e == hipSuccess) e = e1;
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&sl.exec, g, nullptr, nullptr, 0);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    }
+    if (e != hipSuccess) {
+        destroy_slots(s);
+        delete s;
+        return RSORT_ERROR_DEVICE;
+    }
+    for (uint32_t i = 0; i < cfg->depth; ++i) s->free_slots.push_back((int)(cfg->depth - 1 - i));
+    s->scheduler = std::thread([s] { s->run_scheduler(); });
+    s->completer = std::thread([s] { s->run_completer(); });
+    *out = (rsort_server*)s;
+    return RSORT_SUCCESS;
+}
+
+rsort_status rsort_serve_submit(rsort_server* srv, void* keys, void* values, uint64_t n, uint64_t* request_id) {
+    Server* s = (Server*)srv;
+    if (!s || !request_id) return RSORT_ERROR_INVALID_ARGUMENT;
+    if (n > 0 && !keys) return RSORT_ERROR_INVALID_ARGUMENT;
+    if (s->vb && n > 0 && !values) return RSORT_ERROR_INVALID_ARGUMENT;
+    if ((uintptr_t)keys % s->kb || (s->vb && (uintptr_t)values % s->ve)) return RSORT_ERROR_INVALID_ARGUMENT;
+    if (n > s->cfg.batch_keys) return RSORT_ERROR_UNSUPPORTED;  // never fits a batch
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->error != RSORT_SUCCESS) return s->error;
+    if (s->stop) return RSORT_ERROR_BUSY;
+    if (s->cfg.max_pending && s->pending.size() >= s->cfg.max_pending) return RSORT_ERROR_BUSY;
+    const uint64_t id = s->next_id++;
+    rsa::Request r{};
+    r.keys = keys;
+    r.vals = values;
+    r.n = n;
+    r.t_submit = rsa::Clock::now();
+    s->req.emplace(id, r);
+    s->pending.push_back(id);
+    s->pending_keys += n;
+    *request_id = id;
+    s->cv_work.notify_one();
+    return RSORT_SUCCESS;
+}
+
+rsort_status rsort_serve_wait(rsort_server* srv, uint64_t request_id, uint64_t timeout_us, rsort_request_info* info) {
+    Server* s = (Server*)srv;
+    if (!s) return RSORT_ERROR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->req.find(request_id);
+    if (it == s->req.end()) return RSORT_ERROR_INVALID_ARGUMENT;
+    const auto deadline = rsa::Clock::now() + std::chrono::microseconds(timeout_us);
+    while (!it->second.done && s->error == RSORT_SUCCESS) {
+        if (timeout_us == 0 || s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) {
+            if (!it->second.done) return RSORT_ERROR_BUSY;
+        }
+        it = s->req.find(request_id);
+    }
+    if (!it->second.done) return s->error;
+    if (info) {
+        info->latency_us = rsa::us_since(it->second.t_submit, it->second.t_done);
+        info->queue_us = rsa::us_since(it->second.t_submit, it->second.t_launch);
+        info->batch = it->second.batch;
+        info->n = it->second.n;
+    }
+    s->req.erase(it);
+    return RSORT_SUCCESS;
+}
+
+rsort_status rsort_serve_flush(rsort_server* srv) {
+    Server* s = (Server*)srv;
+    if (!s) return RSORT_ERROR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->flush = true;
+    s->cv_work.notify_all();
+    while ((!s->pending.empty() || s->launching || !s->inflight.empty()) && s->error == RSORT_SUCCESS)
+        s->cv_done.wait(lk);
+    return s->error;
+}
+
+rsort_status rsort_serve_stats(rsort_server* srv, rsort_serve_stats_info* out) {
+    Server* s = (Server*)srv;
+    if (!s || !out) return RSORT_ERROR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(s->mu);
+    out->batches = s->batches;
+    out->completed = s->completed;
+    out->completed_keys = s->completed_keys;
+    out->pending = s->pending.size();
+    out->in_flight = s->inflight.size();
+    out->graphs = s->slots.empty() ? 0u : (s->slots[0].exec ? 1u : 0u);
+    out->workspace_bytes = 0;
+    for (auto& sl : s->slots)
+        out->workspace_bytes += sl.ws_bytes + s->cfg.batch_keys * (s->kb + s->vb) + (s->cfg.batch_requests + 1) * 8u;
+    return s->error;
+}
+
+rsort_status rsort_serve_destroy(rsort_server* srv) {
+    Server* s = (Server*)srv;
+    if (!s) return RSORT_ERROR_INVALID_ARGUMENT;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->flush = true;
+        s->cv_work.notify_all();
+        s->cv_slot.notify_all();
+        s->cv_done.notify_all();
+    }
+    if (s->scheduler.joinable()) s->scheduler.join();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->cv_done.notify_all();
+    }
+    if (s->completer.joinable()) s->completer.join();
+    const rsort_status st = s->error;
+    destroy_slots(s);
+    delete s;
+    return st;
+}
+
+}  // extern "C"

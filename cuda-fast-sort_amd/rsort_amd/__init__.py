@@ -78,6 +78,32 @@ class SegPlanInfo(ctypes.Structure):
     ]
 
 
+class ServeConfig(ctypes.Structure):
+    _fields_ = [
+        ("desc", Desc),
+        ("batch_keys", ctypes.c_uint64),
+        ("batch_requests", ctypes.c_uint32),
+        ("window_us", ctypes.c_uint32),
+        ("depth", ctypes.c_uint32),
+        ("max_pending", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("latency_us", ctypes.c_uint64), ("queue_us", ctypes.c_uint64), ("batch", ctypes.c_uint64),
+                ("n", ctypes.c_uint64)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("completed_keys", ctypes.c_uint64),
+                ("pending", ctypes.c_uint64), ("in_flight", ctypes.c_uint64), ("graphs", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("workspace_bytes", ctypes.c_uint64)]
+
+
+SERVE_NO_GRAPH = 1
+
 _lib = None
 
 
@@ -102,6 +128,14 @@ def lib() -> ctypes.CDLL:
         L.rsort_segmented_sort.restype = i32
         L.rsort_segmented_plan.argtypes = [ctypes.POINTER(Desc), u64, u32, u64, ctypes.POINTER(SegPlanInfo)]
         L.rsort_segmented_plan.restype = i32
+        L.rsort_serve_create.argtypes = [ctypes.POINTER(ServeConfig), ctypes.POINTER(vp)]
+        L.rsort_serve_submit.argtypes = [vp, vp, vp, u64, ctypes.POINTER(ctypes.c_uint64)]
+        L.rsort_serve_wait.argtypes = [vp, u64, u64, ctypes.POINTER(RequestInfo)]
+        L.rsort_serve_flush.argtypes = [vp]
+        L.rsort_serve_stats.argtypes = [vp, ctypes.POINTER(ServeStats)]
+        L.rsort_serve_destroy.argtypes = [vp]
+        for f in ("create", "submit", "wait", "flush", "stats", "destroy"):
+            getattr(L, "rsort_serve_" + f).restype = i32
         L.rsort_status_string.argtypes = [i32]
         L.rsort_status_string.restype = ctypes.c_char_p
         L.rsort_last_error.argtypes = []
@@ -256,3 +290,57 @@ def segmented_sort(keys, offsets, temp_bytes, d_workspace, values=None, descendi
     num_segments = max(int(offsets.numel()) - 1, 0)
     return segmented_sort_raw(desc, _ptr(keys), _ptr(values), int(keys.shape[0]), _ptr(offsets),
                               offsets.element_size(), num_segments, _ptr(d_workspace), temp_bytes, stream)
+
+
+# ---------------------------------------------------------------------------
+# Serving loop (no reference counterpart): continuous batching of requests
+# ---------------------------------------------------------------------------
+class Server:
+    """Batches sort requests (device tensors, sorted in place) into segmented
+    sorts on `depth` slots of `batch_keys` keys / `batch_requests` requests."""
+
+    def __init__(self, key_type: KeyType, value_bytes: int = 0, descending: bool = False, batch_keys: int = 1 << 24,
+                 batch_requests: int = 4096, window_us: int = 200, depth: int = 2, max_pending: int = 0,
+                 graphs: bool = True):
+        cfg = ServeConfig()
+        cfg.desc = make_desc(key_type.kind, key_type.nbytes, value_bytes, descending)
+        cfg.batch_keys = batch_keys
+        cfg.batch_requests = batch_requests
+        cfg.window_us = window_us
+        cfg.depth = depth
+        cfg.max_pending = max_pending
+        cfg.flags = 0 if graphs else SERVE_NO_GRAPH
+        h = ctypes.c_void_p()
+        _check(lib().rsort_serve_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.value_bytes = value_bytes
+
+    def submit(self, keys, values=None, n: int | None = None) -> int:
+        rid = ctypes.c_uint64(0)
+        n = int(keys.shape[0]) if n is None else int(n)
+        _check(lib().rsort_serve_submit(self._h, _ptr(keys), _ptr(values), n, ctypes.byref(rid)))
+        return int(rid.value)
+
+    def wait(self, request_id: int, timeout_us: int = 10_000_000) -> RequestInfo:
+        info = RequestInfo()
+        _check(lib().rsort_serve_wait(self._h, request_id, timeout_us, ctypes.byref(info)))
+        return info
+
+    def flush(self) -> None:
+        _check(lib().rsort_serve_flush(self._h))
+
+    def stats(self) -> ServeStats:
+        out = ServeStats()
+        _check(lib().rsort_serve_stats(self._h, ctypes.byref(out)))
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            h, self._h = self._h, None
+            _check(lib().rsort_serve_destroy(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -142,6 +142,59 @@ rsort_status rsort_segmented_plan(const rsort_desc* desc, uint64_t n, uint32_t o
                                   rsort_segmented_plan_info* out);
 
 /*
+ * Serving loop: continuous batching of many small sort requests into
+ * segmented sorts.  No counterpart in the reference (one sort per call); it
+ * is for callers that sort many independent arrays as they arrive.
+ *
+ * A request is a device array of n keys (+ values) of the configured type,
+ * sorted in place like rsort_sort.  A scheduler thread closes a batch when a
+ * slot is full (batch_keys / batch_requests), when its oldest request has
+ * waited window_us, or on flush; the batch runs as gather -> segmented sort
+ * -> scatter on the slot's stream (captured once into a hipGraph unless
+ * RSORT_SERVE_NO_GRAPH).  Admission: a request must fit one slot
+ * (UNSUPPORTED otherwise) and the queue holds at most max_pending requests
+ * (BUSY).  All device memory (depth slots) is allocated at create.
+ * The request's buffers must stay valid until rsort_serve_wait returns.
+ */
+#define RSORT_SERVE_NO_GRAPH 1u
+
+typedef struct rsort_serve_config {
+    rsort_desc desc;          /* element type and order of every request */
+    uint64_t batch_keys;      /* keys per batch slot */
+    uint32_t batch_requests;  /* requests per batch slot */
+    uint32_t window_us;       /* batching window */
+    uint32_t depth;           /* batch slots in flight, 1..8 */
+    uint32_t max_pending;     /* queued requests before submit returns BUSY (0 = unbounded) */
+    uint32_t flags;           /* RSORT_SERVE_* */
+    uint32_t reserved;        /* must be 0 */
+} rsort_serve_config;
+
+typedef struct rsort_server rsort_server;
+
+typedef struct rsort_request_info {
+    uint64_t latency_us; /* submit -> completion observed */
+    uint64_t queue_us;   /* submit -> batch launch */
+    uint64_t batch;      /* batch sequence number */
+    uint64_t n;
+} rsort_request_info;
+
+typedef struct rsort_serve_stats {
+    uint64_t batches, completed, completed_keys, pending, in_flight;
+    uint32_t graphs;     /* 1 when batches replay captured graphs */
+    uint32_t reserved;
+    uint64_t workspace_bytes;
+} rsort_serve_stats_info;
+
+rsort_status rsort_serve_create(const rsort_serve_config* cfg, rsort_server** out);
+rsort_status rsort_serve_submit(rsort_server* s, void* keys, void* values, uint64_t n, uint64_t* request_id);
+/* Blocks until the request is done (BUSY after timeout_us; 0 = poll) and releases its record. */
+rsort_status rsort_serve_wait(rsort_server* s, uint64_t request_id, uint64_t timeout_us, rsort_request_info* info);
+/* Closes the open batch now and waits until nothing is queued or in flight. */
+rsort_status rsort_serve_flush(rsort_server* s);
+rsort_status rsort_serve_stats(rsort_server* s, rsort_serve_stats_info* out);
+rsort_status rsort_serve_destroy(rsort_server* s);
+
+/*
  * Debug builds compiled with -DRSA_PHASE_TIMING accumulate per-phase wall-clock
  * ticks (100 MHz) of the kernels; this copies the summed slots to `out` and
  * returns how many were written (0 in normal builds).  Slots of the pass

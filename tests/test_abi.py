@@ -162,3 +162,16 @@ def test_segmented_undersized_workspace_and_noops(rs):
     # no segments / n <= 1: nothing is launched, no pointer is touched
     rs.segmented_sort_raw(d, 0x1000, 0x2000, 5000, 0x3000, 8, 0, 0x100000, need)
     rs.segmented_sort_raw(d, 0x1000, 0x2000, 1, 0x3000, 8, 7, 0x100000, need)
+
+
+def test_serve_rejects_bad_config_before_device_work(rs):
+    import ctypes
+
+    for field, value in (("depth", 0), ("depth", 9), ("batch_requests", 0), ("batch_keys", 1), ("reserved", 1)):
+        cfg = rs.ServeConfig()
+        cfg.desc = rs.make_desc(0, 4, 4)
+        cfg.batch_keys, cfg.batch_requests, cfg.window_us, cfg.depth = 1 << 20, 64, 100, 2
+        setattr(cfg, field, value)
+        h = ctypes.c_void_p()
+        assert rs.lib().rsort_serve_create(ctypes.byref(cfg), ctypes.byref(h)) == 1, field
+        assert not h.value

@@ -1,0 +1,118 @@
+"""GPU tests of the serving loop: every request's result is identical alone
+(the oracle's sort of that request, which the whole-array GPU sort matches)
+and in the batched mix; admission limits; graph and direct launch paths."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    torch.cuda.init()
+
+
+def _requests(orc, count, seed, max_len=5000, long_every=0):
+    rng = np.random.default_rng(seed)
+    sizes = [int(x) for x in rng.integers(0, max_len, size=count)]
+    if long_every:
+        for i in range(0, count, long_every):
+            sizes[i] = int(rng.integers(16385, 90000))
+    keys = [orc.mt19937(seed * 1000 + i, m) for i, m in enumerate(sizes)]
+    return keys
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+@pytest.mark.parametrize("desc", [False, True])
+def test_alone_equals_mixed(rs, orc, graphs, desc):
+    print(f"[case] serve graphs={graphs} desc={desc}", flush=True)
+    host = _requests(orc, 300, seed=3 + int(desc), long_every=37)
+    dk = [torch.from_numpy(k.copy()).cuda() for k in host]
+    dv = [torch.arange(k.size, dtype=torch.int32, device="cuda") for k in host]
+    torch.cuda.synchronize()
+    with rs.Server(rs.KeyType(0, 4), value_bytes=4, descending=desc, batch_keys=1 << 20, batch_requests=64,
+                   window_us=100, depth=3, graphs=graphs) as srv:
+        ids = [srv.submit(k, v) for k, v in zip(dk, dv)]
+        infos = [srv.wait(i) for i in ids]
+        st = srv.stats()
+        assert st.completed == len(ids) and st.pending == 0 and st.graphs == int(graphs)
+        assert st.batches >= 2  # 300 requests cannot share one 64-request slot
+    for k, v, d_k, d_v, info in zip(host, [np.arange(k.size, dtype=np.int32) for k in host], dk, dv, infos):
+        ek, ev = orc.sort(k, v, 0, 4, desc, vb=4)
+        assert np.array_equal(d_k.cpu().numpy(), ek)
+        assert np.array_equal(d_v.cpu().numpy(), ev)
+        assert info.n == k.size and info.latency_us >= info.queue_us
+
+
+def test_same_request_alone_and_in_mix(rs, orc):
+    """One request served alone (a batch of one) and inside a mix of 200."""
+    probe = orc.mt19937(99, 7777)
+    others = _requests(orc, 200, seed=5)
+    with rs.Server(rs.KeyType(0, 4), batch_keys=1 << 21, batch_requests=256, window_us=50, depth=2) as srv:
+        a = torch.from_numpy(probe.copy()).cuda()
+        srv.wait(srv.submit(a))
+        b = torch.from_numpy(probe.copy()).cuda()
+        dothers = [torch.from_numpy(k.copy()).cuda() for k in others]
+        ids = [srv.submit(x) for x in dothers[:100]] + [srv.submit(b)] + [srv.submit(x) for x in dothers[100:]]
+        infos = {i: srv.wait(i) for i in ids}
+        assert infos[ids[100]].batch != 0
+    assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    assert np.array_equal(a.cpu().numpy(), np.sort(probe))
+
+
+def test_key_kinds_and_values(rs, orc):
+    from helpers import random_keys, random_values
+
+    for kind, kb, vb in ((2, 4, 8), (1, 8, 4), (0, 2, 0), (2, 8, 12)):
+        print(f"[case] serve kind={kind} kb={kb} vb={vb}", flush=True)
+        rng = np.random.default_rng(kb + vb)
+        sizes = [int(x) for x in rng.integers(1, 20000, size=40)]
+        host_k = [random_keys(kind, kb, m, seed=i) for i, m in enumerate(sizes)]
+        host_v = [random_values(vb, m, seed=i) if vb else None for i, m in enumerate(sizes)]
+        dk = [torch.from_numpy(k.copy()).cuda() for k in host_k]
+        dv = [torch.from_numpy(v.copy()).cuda() if v is not None else None for v in host_v]
+        with rs.Server(rs.KeyType(kind, kb), value_bytes=vb, batch_keys=1 << 19, batch_requests=32,
+                       window_us=20, depth=2) as srv:
+            ids = [srv.submit(k, v, n=m) for k, v, m in zip(dk, dv, sizes)]
+            for i in ids:
+                srv.wait(i)
+        for k, v, d_k, d_v in zip(host_k, host_v, dk, dv):
+            ek, ev = orc.sort(k, v, kind, kb, False, vb=vb if vb else None)
+            assert np.array_equal(d_k.cpu().numpy(), ek)
+            if vb:
+                assert np.array_equal(d_v.cpu().numpy(), ev)
+
+
+def test_admission_limits(rs):
+    with rs.Server(rs.KeyType(0, 4), batch_keys=4096, batch_requests=8, window_us=1_000_000, depth=1,
+                   max_pending=4) as srv:
+        big = torch.zeros(5000, dtype=torch.int32, device="cuda")
+        with pytest.raises(rs.RsortError) as e:
+            srv.submit(big)  # larger than a batch slot
+        assert e.value.status == 2
+        small = [torch.randint(0, 100, (10,), dtype=torch.int32, device="cuda") for _ in range(5)]
+        ids = [srv.submit(x) for x in small[:4]]
+        with pytest.raises(rs.RsortError) as e:
+            srv.submit(small[4])  # queue full: the window holds the batch open
+        assert e.value.status == 5
+        srv.flush()
+        for i in ids:
+            assert srv.wait(i, timeout_us=0).n == 10
+        for x in small[:4]:
+            v = x.cpu().numpy()
+            assert np.all(v[1:] >= v[:-1])
+
+
+def test_flush_empty_and_zero_length(rs):
+    with rs.Server(rs.KeyType(0, 4), batch_keys=1 << 16, batch_requests=16, window_us=10, depth=2) as srv:
+        srv.flush()
+        z = torch.empty(0, dtype=torch.int32, device="cuda")
+        one = torch.tensor([5], dtype=torch.int32, device="cuda")
+        i0, i1 = srv.submit(z), srv.submit(one)
+        assert srv.wait(i0).n == 0 and srv.wait(i1).n == 1
+        assert one.item() == 5
