@@ -29,9 +29,17 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["kv", "keys", "copy"], default="kv",
+    ap.add_argument("--workload", choices=["kv", "keys", "copy", "segments", "serve"], default="kv",
                     help="kv: sort_pairs u32/u32; keys: sort u32; copy: device copy of the keys "
-                         "(stream-copy bandwidth, and the known-size kernel that calibrates HBM counters)")
+                         "(stream-copy bandwidth, and the known-size kernel that calibrates HBM counters); "
+                         "segments: one segmented sort of 100k lognormal-sized u32/u32 segments vs one "
+                         "sort_pairs call per segment; serve: Poisson request stream through the serving loop")
+    ap.add_argument("--segments", type=int, default=100_000)
+    ap.add_argument("--per-call-sample", type=int, default=5000,
+                    help="segments sorted one sort_pairs call each for the per-call comparison")
+    ap.add_argument("--rate", type=float, default=5000.0, help="serve: Poisson arrival rate, requests/s")
+    ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds of arrivals")
+    ap.add_argument("--window-us", type=int, default=200, help="serve: batching window")
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--descending", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -108,6 +116,193 @@ def copy_bandwidth(n, steps):
     return 2.0 * n * 4 / (ms * 1e-3) / 1e9, ms
 
 
+def lognormal_sizes(count, seed=5, median=2048.0, sigma=0.7356, lo=32, hi=1 << 22):
+    """Request / segment sizes: lognormal with median 2048 and sigma 0.7356
+    (mean ~2684 keys), rounded and clipped to [32, 2^22]."""
+    import numpy as np
+
+    rng = np.random.Generator(np.random.MT19937(seed))
+    v = np.floor(median * np.exp(sigma * rng.standard_normal(count)) + 0.5)
+    return np.clip(v, lo, hi).astype(np.int64)
+
+
+def bench_segments(rs, a):
+    """One segmented sort of the whole batch per step vs one sort_pairs call per segment."""
+    import numpy as np
+    import torch
+
+    dev = torch.device("cuda:0")
+    sizes = lognormal_sizes(a.segments)
+    n = int(sizes.sum())
+    offs_np = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    offsets = torch.from_numpy(offs_np).to(dev)
+    keys0 = gen_u32(n, dev, 7)
+    vals0 = torch.arange(n, dtype=torch.int32, device=dev)
+    desc = rs.make_desc(rs.KEY_UNSIGNED, 4, 4, a.descending)
+    tb = rs.segmented_sort_raw(desc, None, None, n, None, 8, a.segments, None, 0)
+    ws = torch.empty(tb, dtype=torch.uint8, device=dev)
+    total = a.steps + a.warmup
+    ks = [keys0.clone() for _ in range(total)]
+    vs = [vals0.clone() for _ in range(total)]
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    call = lambda i: rs.segmented_sort_raw(desc, ks[i].data_ptr(), vs[i].data_ptr(), n, offsets.data_ptr(), 8,
+                                           a.segments, ws.data_ptr(), tb, stream)
+    for i in range(a.warmup):
+        call(i)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps + 1)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for j, i in enumerate(range(a.warmup, total)):
+        call(i)
+        ev[j + 1].record(stream)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    dev_ms = [ev[j].elapsed_time(ev[j + 1]) for j in range(a.steps)]
+    ms = sum(dev_ms) / len(dev_ms)
+    # every segment sorted, and values still pair with their keys
+    k = ks[-1].view(torch.int32) ^ (-(2**31))
+    up = k[1:] >= k[:-1]
+    seg_start = torch.zeros(n, dtype=torch.bool, device=dev)
+    seg_start[offsets[1:-1]] = True
+    ok = bool((up | seg_start[1:]).all().item()) and bool((keys0[vs[-1].long()] == ks[-1]).all().item())
+    lds_cap = rs.segmented_plan(desc, n, 8, a.segments).bin_capacity[4]
+    small = sizes[sizes <= lds_cap].sum()
+    big = sizes[sizes > lds_cap].sum()
+    kvb = 8
+    bytes_ = 2.0 * small * kvb + 5 * 2.0 * big * kvb  # LDS segments: one read + one write; long: formula
+    batched_k, batched_v = ks[-1], vs[-1]
+    del ks, vs
+    # per-call comparison: the same segments, one sort_pairs call each
+    m = min(a.per_call_sample, a.segments)
+    pk, pv = keys0.clone(), vals0.clone()
+    fdesc = rs.make_desc(rs.KEY_UNSIGNED, 4, 4, a.descending)
+    ftb = rs.sort_raw(fdesc, None, None, None, 0, int(sizes[:m].max()))
+    fws = torch.empty(ftb, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kp, vp = pk.data_ptr(), pv.data_ptr()
+    for i in range(m):
+        o, c = int(offs_np[i]), int(sizes[i])
+        rs.sort_raw(fdesc, kp + 4 * o, vp + 4 * o, fws.data_ptr(), ftb, c, stream)
+    torch.cuda.synchronize()
+    per_call_s = time.perf_counter() - t0
+    per_call_keys = int(sizes[:m].sum())
+    same = bool((pk[:per_call_keys] == batched_k[:per_call_keys]).all().item()) and bool(
+        (pv[:per_call_keys] == batched_v[:per_call_keys]).all().item())
+    batched = n / (ms * 1e-3) / 1e9
+    per_call = per_call_keys / per_call_s / 1e9
+    return {
+        "metric": "segmented sort keys/s, 100k lognormal segments vs per-call sort_pairs",
+        "value": round(batched, 3), "unit": "GKeys/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms, 4), "wall_ms_per_step": round(wall_ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"segmented sort_pairs u32/u32, {a.segments} segments, lognormal sizes median 2048 "
+                               f"sigma 0.7356 clipped [32, 2^22], n={n}", "segments": a.segments, "n": n,
+                   "lds_capacity": int(lds_cap), "keys_in_lds_segments": int(small), "keys_in_long_segments": int(big)},
+        "roofline": {"bound": "hbm", "kernel": "whole segmented sort (device time)",
+                     "achieved": round(bytes_ / (ms * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(bytes_ / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "traffic": None,
+                     "bytes": bytes_},
+        "per_call": {"gkeys_per_s": round(per_call, 5), "calls": m, "keys": per_call_keys,
+                     "us_per_call": round(per_call_s * 1e6 / m, 2),
+                     "note": "one rsort_sort call (sort_pairs) per segment from Python via ctypes, same stream"},
+        "batched_over_per_call": round(batched / per_call, 1),
+        "checked": ok,
+        "per_call_equals_batched": same,
+    }
+
+
+def bench_serve(rs, a):
+    """Poisson request stream through the serving loop: latency percentiles and
+    throughput; then the same requests submitted at once (saturation)."""
+    import numpy as np
+    import torch
+
+    dev = torch.device("cuda:0")
+    count = max(1, int(a.rate * a.duration))
+    sizes = lognormal_sizes(count, seed=11)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(offs[-1])
+    keys = gen_u32(n, dev, 13)
+    vals = torch.arange(n, dtype=torch.int32, device=dev)
+    keys_ref = keys.clone()
+    kp, vp = keys.data_ptr(), vals.data_ptr()
+    torch.cuda.synchronize()
+    srv = rs.Server(rs.KeyType(rs.KEY_UNSIGNED, 4), value_bytes=4, batch_keys=1 << 24, batch_requests=8192,
+                    window_us=a.window_us, depth=3)
+    # warm-up requests (not timed)
+    wk, wv = keys_ref.clone(), vals.clone()
+    warm = [_submit_raw(rs, srv, wk.data_ptr() + 4 * int(offs[i]), wv.data_ptr() + 4 * int(offs[i]), int(sizes[i]))
+            for i in range(min(50, count))]
+    for w in warm:
+        srv.wait(w)
+    del wk, wv
+    keys.copy_(keys_ref)
+    vals.copy_(torch.arange(n, dtype=torch.int32, device=dev))
+    torch.cuda.synchronize()
+    st0 = srv.stats()
+    rng = np.random.default_rng(17)
+    gaps = rng.exponential(1.0 / a.rate, size=count)
+    ids = []
+    t_start = time.perf_counter()
+    t_next = t_start
+    for i in range(count):
+        t_next += gaps[i]
+        while time.perf_counter() < t_next:
+            pass
+        o = int(offs[i])
+        ids.append(_submit_raw(rs, srv, kp + 4 * o, vp + 4 * o, int(sizes[i])))
+    infos = [srv.wait(i) for i in ids]
+    t_end = time.perf_counter()
+    lat = np.array([x.latency_us for x in infos], dtype=np.float64)
+    q = np.array([x.queue_us for x in infos], dtype=np.float64)
+    st = srv.stats()
+    # correctness: every request sorted with its values
+    k = keys.view(torch.int32) ^ (-(2**31))
+    starts = torch.zeros(n, dtype=torch.bool, device=dev)
+    starts[torch.from_numpy(offs[1:-1]).to(dev)] = True
+    ok = bool(((k[1:] >= k[:-1]) | starts[1:]).all().item()) and bool((keys_ref[vals.long()] == keys).all().item())
+    # saturation: the same requests all at once
+    keys.copy_(keys_ref)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ids2 = [_submit_raw(rs, srv, kp + 4 * int(offs[i]), vp + 4 * int(offs[i]), int(sizes[i])) for i in range(count)]
+    for i in ids2:
+        srv.wait(i)
+    t_sat = time.perf_counter() - t0
+    st2 = srv.stats()
+    srv.close()
+    return {
+        "metric": "serving loop latency p50/p99 at a Poisson request rate, lognormal request sizes",
+        "value": round(float(np.percentile(lat, 99)), 1), "unit": "us (p99 latency)", "n_gpus": 1,
+        "higher_is_better": False, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"sort_pairs u32/u32 requests, Poisson {a.rate:g} req/s for {a.duration:g} s, sizes "
+                               "lognormal median 2048 sigma 0.7356 clipped [32, 2^22]", "requests": count,
+                   "keys": n, "batch_keys": 1 << 24, "batch_requests": 8192, "window_us": a.window_us, "depth": 3,
+                   "graphs": int(st.graphs)},
+        "latency_us": {"p50": round(float(np.percentile(lat, 50)), 1), "p90": round(float(np.percentile(lat, 90)), 1),
+                       "p99": round(float(np.percentile(lat, 99)), 1), "max": round(float(lat.max()), 1),
+                       "mean": round(float(lat.mean()), 1), "queue_p50": round(float(np.percentile(q, 50)), 1)},
+        "offered_keys_per_s": round(n / a.duration / 1e9, 5),
+        "achieved_keys_per_s": round(n / (t_end - t_start) / 1e9, 5),
+        "batches": int(st.batches - st0.batches),
+        "mean_requests_per_batch": round(count / max(1, int(st.batches - st0.batches)), 2),
+        "saturation": {"gkeys_per_s": round(n / t_sat / 1e9, 3), "seconds": round(t_sat, 4),
+                       "batches": int(st2.batches - st.batches)},
+        "checked": ok,
+    }
+
+
+def _submit_raw(rs, srv, kptr, vptr, n):
+    import ctypes
+
+    rid = ctypes.c_uint64(0)
+    rs._check(rs.lib().rsort_serve_submit(srv._h, kptr, vptr, n, ctypes.byref(rid)))
+    return int(rid.value)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -120,6 +315,12 @@ def main():
     rs.lib()
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     n = 1 << a.log2n
+    if a.workload == "segments":
+        print(json.dumps(bench_segments(rs, a)), flush=True)
+        return
+    if a.workload == "serve":
+        print(json.dumps(bench_serve(rs, a)), flush=True)
+        return
     if a.workload == "copy":
         gbs, ms = copy_bandwidth(n, a.steps)
         print(json.dumps({"metric": "device copy bandwidth", "value": round(gbs, 1), "unit": "GB/s",

@@ -371,6 +371,7 @@ rsort_status run_segmented(const rsort_desc* d, void* keys, void* values, uint64
     j.max_large = p.max_large;
     j.max_tiles = p.max_tiles;
     j.num_cus = cus;
+    j.dbg = debug_slots();
     j.events = events;
     j.max_events = max_events;
     j.n_events = n_events;

@@ -66,6 +66,7 @@ struct SegJob {
     uint64_t max_large;   // bound on long segments
     uint64_t max_tiles;   // bound on their tiles
     int num_cus;
+    unsigned long long* dbg;  // phase-timing slots (RSA_PHASE_TIMING builds)
     hipEvent_t* events;
     int max_events;
     int* n_events;

@@ -17,8 +17,8 @@ int blocks_per_cu(F kernel, int threads) {
 
 template <int KB, int VB, int BIN>
 hipError_t launch_seg_bin(const SegParams& a, const SegJob& j, hipStream_t s) {
-    constexpr int THREADS = 64 << BIN;
-    constexpr int K = (int)seg_items(KB, VB);
+    constexpr int THREADS = (int)seg_bin_threads(KB, VB, BIN);
+    constexpr int K = (int)seg_bin_items(KB, VB, BIN);
     static const int occ = blocks_per_cu(k_seg_lds<KB, VB, THREADS, K>, THREADS);
     uint64_t grid = (uint64_t)j.num_cus * (uint64_t)occ;
     if (grid > j.num_segments) grid = j.num_segments;
@@ -57,6 +57,7 @@ hipError_t launch_seg_t(const SegJob& j, hipStream_t s) {
     a.tile = C.threads * C.items;
     for (int b = 0; b < kSegBins; ++b) a.cap[b] = seg_cap(KB, VB, b);
     a.tw = j.tw;
+    a.dbg = j.dbg;
 
     // 1. classification and queues
     record_event(j, s);

@@ -54,6 +54,7 @@ struct SegParams {
     uint32_t cap[kSegBins];  // LDS bin capacities (keys)
     uint32_t pad;
     TwMasks tw;
+    unsigned long long* dbg;  // phase-timing slots (RSA_PHASE_TIMING builds), else null
 };
 
 // Segment s as a clamped, non-negative range [b, e) of [0, n).
@@ -226,24 +227,38 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
     KM* __restrict__ keys = (KM*)a.keys;
     VM* __restrict__ vals = (VM*)a.vals;
     const uint32_t wl = w * 64u * K + lane;  // tile-local index of item 0
+    RSA_TIMER_START(a.dbg);
 
     for (;;) {
         if (tid == 0) s_item = atomicAdd(&a.ctrl->bin_ticket[bin], 1u);
         __syncthreads();
         const uint32_t item = s_item;
         if (item >= count) break;
+        RSA_MARK(22);
         uint64_t sb, se;
         seg_range(a, q[item], sb, se);
         const uint32_t m = (uint32_t)(se - sb);
 
+        // unconditional loads (slots past the end re-read the last key) so
+        // that all of them are in flight at once; fillers are chosen after
         KR key[K];
         VM val[VB > 0 ? K : 1];
 #pragma unroll
         for (int i = 0; i < K; ++i) {
             const uint32_t idx = wl + (uint32_t)i * 64u;
-            key[i] = idx < m ? tw_in<KB>(key_to_reg<KB>(keys[sb + idx]), a.tw) : key_filler<KB>();
-            if constexpr (VB > 0) val[i] = idx < m ? vals[sb + idx] : VM{};
+            const uint64_t at = sb + (idx < m ? idx : m - 1u);
+            key[i] = key_to_reg<KB>(keys[at]);
+            if constexpr (VB > 0) val[i] = vals[at];
         }
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const uint32_t idx = wl + (uint32_t)i * 64u;
+            key[i] = idx < m ? tw_in<KB>(key[i], a.tw) : key_filler<KB>();
+        }
+#ifdef RSA_PHASE_TIMING
+        __builtin_amdgcn_s_waitcnt(0);
+#endif
+        RSA_MARK(23);
 
         for (uint32_t p = 0; p < (uint32_t)KB; ++p) {
             // the previous exchange has been read back (barrier at its end)
@@ -260,6 +275,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
             rank_wave<KB, K>(key, 8u * (p & 3u), p >> 2, (uint64_t*)s_big + (size_t)w * 256, s_cnt + w * 256, lane,
                              dr);
             __syncthreads();
+            RSA_MARK(24);
 
             // digit scan: per digit, exclusive over waves; then exclusive over
             // digits; s_cnt[w][d] becomes the start of wave w's run of digit d
@@ -295,14 +311,15 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
                 }
             }
             __syncthreads();
+            RSA_MARK(25);
             if (s_uniform[p & 1u]) continue;  // one digit: this pass keeps the order
-
             KM* SK = (KM*)s_big;
 #pragma unroll
             for (int i = 0; i < K; ++i) SK[s_cnt[w * 256 + (dr[i] & 255u)] + (dr[i] >> 8)] = key_to_mem<KB>(key[i]);
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < K; ++i) key[i] = key_to_reg<KB>(SK[wl + (uint32_t)i * 64u]);
+            RSA_MARK(26);
             if constexpr (VB > 0) {
                 __syncthreads();
                 VM* SV = (VM*)s_big;
@@ -313,6 +330,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
                 for (int i = 0; i < K; ++i) val[i] = SV[wl + (uint32_t)i * 64u];
             }
             __syncthreads();
+            RSA_MARK(27);
         }
 
 #pragma unroll
@@ -325,7 +343,10 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
         }
         // s_item and the LDS tables are reused by the next segment
         __syncthreads();
+        RSA_MARK(28);
+        if (tid == 0) RSA_COUNT(29, 1u);
     }
+    RSA_TOTAL(30);
 }
 
 // ---------------------------------------------------------------------------

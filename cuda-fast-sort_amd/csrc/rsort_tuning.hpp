@@ -113,8 +113,13 @@ RSA_HD constexpr uint32_t seg_items(int key_bytes, int val_bytes) {
     return wide <= 4 ? 16u : (wide == 8 ? 8u : 4u);
 }
 
+// Bin b: threads and items per thread.  (Measured on MI355X: twice the
+// threads at half the items per bin, for more waves in flight, was slower.)
+RSA_HD constexpr uint32_t seg_bin_threads(int, int, int bin) { return 64u << bin; }
+RSA_HD constexpr uint32_t seg_bin_items(int key_bytes, int val_bytes, int) { return seg_items(key_bytes, val_bytes); }
+
 RSA_HD constexpr uint32_t seg_cap(int key_bytes, int val_bytes, int bin) {
-    return seg_items(key_bytes, val_bytes) * (64u << bin);
+    return seg_bin_items(key_bytes, val_bytes, bin) * seg_bin_threads(key_bytes, val_bytes, bin);
 }
 
 }  // namespace rsa
