@@ -98,6 +98,22 @@ def run_sort_bench(rs, n, with_values, descending, steps, warmup, seed=1):
     return (t1 - t0) * 1e3 / steps, stages, ok, keys0, vals0
 
 
+def measured_traffic(kernel_prefix, n):
+    """HBM bytes per launch of the pass kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE and WRITE_SIZE, calibrated against a copy of known
+    size), when it was taken on this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "onesweep_kv_u32_2p28_hbm_traffic.md")
+    try:
+        text = open(path).read()
+        block = text[text.index("```json") + 7:]
+        d = json.loads(block[:block.index("```")])
+    except (OSError, ValueError):
+        return None
+    if d.get("n") == n and d.get("kernel", "").startswith(kernel_prefix):
+        return d["traffic_bytes"]
+    return None
+
+
 def copy_bandwidth(n, steps):
     """Device-to-device copy of n u32 (read n*4 + write n*4 bytes); returns GB/s."""
     import torch
@@ -363,7 +379,8 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": "k_onesweep (one LSD pass)",
             "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+            "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": measured_traffic("k_onesweep<4,4," if kv else "k_onesweep<4,0,", n),
             "bytes_per_launch": pass_bytes, "avg_launch_ms": round(avg_pass, 5),
         },
         "formula_roofline": {
