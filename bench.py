@@ -39,7 +39,9 @@ def parse():
                     help="segments sorted one sort_pairs call each for the per-call comparison")
     ap.add_argument("--rate", type=float, default=5000.0, help="serve: Poisson arrival rate, requests/s")
     ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds of arrivals")
-    ap.add_argument("--window-us", type=int, default=200, help="serve: batching window")
+    ap.add_argument("--window-us", type=int, default=0,
+                    help="serve: batching window (0: launch as soon as a batch slot is free)")
+    ap.add_argument("--serve-mode", choices=["direct", "graph", "batched"], default="direct")
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--descending", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -247,7 +249,7 @@ def bench_serve(rs, a):
     kp, vp = keys.data_ptr(), vals.data_ptr()
     torch.cuda.synchronize()
     srv = rs.Server(rs.KeyType(rs.KEY_UNSIGNED, 4), value_bytes=4, batch_keys=1 << 24, batch_requests=8192,
-                    window_us=a.window_us, depth=3)
+                    window_us=a.window_us, depth=3, mode=a.serve_mode)
     # warm-up requests (not timed)
     wk, wv = keys_ref.clone(), vals.clone()
     warm = [_submit_raw(rs, srv, wk.data_ptr() + 4 * int(offs[i]), wv.data_ptr() + 4 * int(offs[i]), int(sizes[i]))
@@ -280,11 +282,14 @@ def bench_serve(rs, a):
     starts = torch.zeros(n, dtype=torch.bool, device=dev)
     starts[torch.from_numpy(offs[1:-1]).to(dev)] = True
     ok = bool(((k[1:] >= k[:-1]) | starts[1:]).all().item()) and bool((keys_ref[vals.long()] == keys).all().item())
-    # saturation: the same requests all at once
+    # saturation: the same requests all at once (one submit_many call)
     keys.copy_(keys_ref)
+    kptrs = [kp + 4 * int(offs[i]) for i in range(count)]
+    vptrs = [vp + 4 * int(offs[i]) for i in range(count)]
+    ns = [int(x) for x in sizes]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ids2 = [_submit_raw(rs, srv, kp + 4 * int(offs[i]), vp + 4 * int(offs[i]), int(sizes[i])) for i in range(count)]
+    ids2 = srv.submit_many(kptrs, vptrs, ns)
     for i in ids2:
         srv.wait(i)
     t_sat = time.perf_counter() - t0
@@ -297,7 +302,7 @@ def bench_serve(rs, a):
         "config": {"workload": f"sort_pairs u32/u32 requests, Poisson {a.rate:g} req/s for {a.duration:g} s, sizes "
                                "lognormal median 2048 sigma 0.7356 clipped [32, 2^22]", "requests": count,
                    "keys": n, "batch_keys": 1 << 24, "batch_requests": 8192, "window_us": a.window_us, "depth": 3,
-                   "graphs": int(st.graphs)},
+                   "mode": a.serve_mode},
         "latency_us": {"p50": round(float(np.percentile(lat, 50)), 1), "p90": round(float(np.percentile(lat, 90)), 1),
                        "p99": round(float(np.percentile(lat, 99)), 1), "max": round(float(lat.max()), 1),
                        "mean": round(float(lat.mean()), 1), "queue_p50": round(float(np.percentile(q, 50)), 1)},

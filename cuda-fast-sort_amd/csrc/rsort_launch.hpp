@@ -72,6 +72,17 @@ struct SegJob {
     int* n_events;
 };
 
+// Requests sorted in place by the LDS bins, one workgroup each (the serving
+// loop's direct form): reqs = SegReq list grouped by bin, count[b] of bin b
+// starting at first[b].  Every request fits its bin (n <= bin capacity).
+struct ReqJob {
+    uint32_t kb, vb;
+    TwMasks tw;
+    const void* reqs;
+    uint32_t first[kSegBins];
+    uint32_t count[kSegBins];
+};
+
 // One launcher per key width, each in its own translation unit.
 hipError_t launch_core_kb1(const SortJob& j, hipStream_t s);
 hipError_t launch_core_kb2(const SortJob& j, hipStream_t s);
@@ -79,6 +90,11 @@ hipError_t launch_core_kb4(const SortJob& j, hipStream_t s);
 hipError_t launch_core_kb8(const SortJob& j, hipStream_t s);
 hipError_t launch_core_kb16(const SortJob& j, hipStream_t s);
 
+hipError_t launch_req_kb1(const ReqJob& j, hipStream_t s);
+hipError_t launch_req_kb2(const ReqJob& j, hipStream_t s);
+hipError_t launch_req_kb4(const ReqJob& j, hipStream_t s);
+hipError_t launch_req_kb8(const ReqJob& j, hipStream_t s);
+hipError_t launch_req_kb16(const ReqJob& j, hipStream_t s);
 hipError_t launch_seg_kb1(const SegJob& j, hipStream_t s);
 hipError_t launch_seg_kb2(const SegJob& j, hipStream_t s);
 hipError_t launch_seg_kb4(const SegJob& j, hipStream_t s);

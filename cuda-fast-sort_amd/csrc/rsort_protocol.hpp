@@ -50,6 +50,15 @@ struct SegCtrl {
 };
 static_assert(sizeof(SegCtrl) == 256, "segmented control header must stay 256 bytes");
 
+// One request of the direct (pointer-list) form of the segmented LDS bins:
+// its own key and value arrays, sorted in place.
+struct SegReq {
+    uint64_t keys;  // device address
+    uint64_t vals;  // device address (0 without values)
+    uint64_t n;
+    uint64_t pad;
+};
+
 // ---------------------------------------------------------------------------
 // Lookback words.  One word per (tile, digit).  Flag, epoch and value share
 // one machine word so a single atomic load observes a consistent triple.

@@ -19,11 +19,49 @@ template <int KB, int VB, int BIN>
 hipError_t launch_seg_bin(const SegParams& a, const SegJob& j, hipStream_t s) {
     constexpr int THREADS = (int)seg_bin_threads(KB, VB, BIN);
     constexpr int K = (int)seg_bin_items(KB, VB, BIN);
-    static const int occ = blocks_per_cu(k_seg_lds<KB, VB, THREADS, K>, THREADS);
+    static const int occ = blocks_per_cu(k_seg_lds<KB, VB, THREADS, K, false>, THREADS);
     uint64_t grid = (uint64_t)j.num_cus * (uint64_t)occ;
     if (grid > j.num_segments) grid = j.num_segments;
-    hipLaunchKernelGGL((k_seg_lds<KB, VB, THREADS, K>), dim3((uint32_t)grid), dim3(THREADS), 0, s, a, (uint32_t)BIN);
+    hipLaunchKernelGGL((k_seg_lds<KB, VB, THREADS, K, false>), dim3((uint32_t)grid), dim3(THREADS), 0, s, a,
+                       (uint32_t)BIN);
     return hipGetLastError();
+}
+
+// Direct form: count[b] requests of bin b at reqs[first[b] ..], one
+// workgroup each.
+template <int KB, int VB, int BIN>
+void launch_req_bin(const SegParams& a, const uint32_t* first, const uint32_t* count, hipStream_t s) {
+    constexpr int THREADS = (int)seg_bin_threads(KB, VB, BIN);
+    constexpr int K = (int)seg_bin_items(KB, VB, BIN);
+    if (count[BIN])
+        hipLaunchKernelGGL((k_seg_lds<KB, VB, THREADS, K, true>), dim3(count[BIN]), dim3(THREADS), 0, s, a,
+                           first[BIN]);
+}
+
+template <int KB, int VB>
+hipError_t launch_req_v(const ReqJob& j, hipStream_t s) {
+    SegParams a{};
+    a.reqs = (const SegReq*)j.reqs;
+    a.tw = j.tw;
+    launch_req_bin<KB, VB, 0>(a, j.first, j.count, s);
+    launch_req_bin<KB, VB, 1>(a, j.first, j.count, s);
+    launch_req_bin<KB, VB, 2>(a, j.first, j.count, s);
+    launch_req_bin<KB, VB, 3>(a, j.first, j.count, s);
+    launch_req_bin<KB, VB, 4>(a, j.first, j.count, s);
+    return hipGetLastError();
+}
+
+template <int KB>
+hipError_t launch_req(const ReqJob& j, hipStream_t s) {
+    switch (j.vb) {
+        case 0: return launch_req_v<KB, 0>(j, s);
+        case 1: return launch_req_v<KB, 1>(j, s);
+        case 2: return launch_req_v<KB, 2>(j, s);
+        case 4: return launch_req_v<KB, 4>(j, s);
+        case 8: return launch_req_v<KB, 8>(j, s);
+        case 16: return launch_req_v<KB, 16>(j, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 template <int KB, int VB, bool WIDE>

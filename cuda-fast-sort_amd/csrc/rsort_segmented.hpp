@@ -55,6 +55,7 @@ struct SegParams {
     uint32_t pad;
     TwMasks tw;
     unsigned long long* dbg;  // phase-timing slots (RSA_PHASE_TIMING builds), else null
+    const SegReq* reqs;       // direct form: request list (may be host-mapped memory)
 };
 
 // Segment s as a clamped, non-negative range [b, e) of [0, n).
@@ -197,7 +198,9 @@ static __global__ __launch_bounds__(kSegChunk) void k_seg_fill(SegParams a) {
 // ---------------------------------------------------------------------------
 // LDS bins: one workgroup sorts one segment of at most THREADS*K keys.
 // ---------------------------------------------------------------------------
-template <int KB, int VB, int THREADS, int K>
+// DIRECT: workgroup b sorts request reqs[bin + b] (grid = the bin's request
+// count); otherwise a persistent workgroup takes segments of bin `bin` by ticket.
+template <int KB, int VB, int THREADS, int K, bool DIRECT>
 __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) {
     using KM = typename KeyTraits<KB>::Mem;
     using KR = typename KeyTraits<KB>::Reg;
@@ -222,22 +225,35 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t w = tid >> 6;
-    const uint32_t count = a.ctrl->cat_count[bin];
-    const uint32_t* __restrict__ q = a.queue + a.ctrl->cat_base[bin];
-    KM* __restrict__ keys = (KM*)a.keys;
-    VM* __restrict__ vals = (VM*)a.vals;
+    const uint32_t count = DIRECT ? 0u : a.ctrl->cat_count[bin];
+    const uint32_t* __restrict__ q = DIRECT ? nullptr : a.queue + a.ctrl->cat_base[bin];
     const uint32_t wl = w * 64u * K + lane;  // tile-local index of item 0
     RSA_TIMER_START(a.dbg);
 
-    for (;;) {
-        if (tid == 0) s_item = atomicAdd(&a.ctrl->bin_ticket[bin], 1u);
-        __syncthreads();
-        const uint32_t item = s_item;
-        if (item >= count) break;
+    for (uint32_t round = 0;; ++round) {
+        KM* __restrict__ keys;
+        VM* __restrict__ vals;
+        uint64_t sb;
+        uint32_t m;
+        if constexpr (DIRECT) {
+            if (round) break;
+            const SegReq r = a.reqs[bin + blockIdx.x];
+            keys = (KM*)r.keys;
+            vals = (VM*)r.vals;
+            sb = 0;
+            m = (uint32_t)r.n;
+        } else {
+            if (tid == 0) s_item = atomicAdd(&a.ctrl->bin_ticket[bin], 1u);
+            __syncthreads();
+            const uint32_t item = s_item;
+            if (item >= count) break;
+            uint64_t se;
+            seg_range(a, q[item], sb, se);
+            m = (uint32_t)(se - sb);
+            keys = (KM*)a.keys;
+            vals = (VM*)a.vals;
+        }
         RSA_MARK(22);
-        uint64_t sb, se;
-        seg_range(a, q[item], sb, se);
-        const uint32_t m = (uint32_t)(se - sb);
 
         // unconditional loads (slots past the end re-read the last key) so
         // that all of them are in flight at once; fillers are chosen after

@@ -11,12 +11,17 @@
 //                 oldest request has waited `window_us`, or on flush, and
 //                 launches it on a free slot (one stream per slot, `depth`
 //                 slots in flight).
-//   batch         k_serve_gather copies every request into the slot's device
-//                 buffer (copy list read from host-mapped memory) and writes
-//                 the segment offsets; the segmented sort sorts all requests;
-//                 k_serve_scatter copies the results back.  The three steps
-//                 are captured once per slot into a hipGraph (fixed capacity
-//                 => fixed launch parameters) and replayed for every batch.
+//   batch         direct form (default): the host knows every request's size,
+//                 so it bins them itself; each non-empty LDS bin is one launch
+//                 whose workgroups sort one request each, in place, reading
+//                 the request list from host-mapped memory; a request longer
+//                 than the LDS capacity gets its own rsort_sort on the slot's
+//                 stream.  Batched form (RSORT_SERVE_BATCHED): k_serve_gather
+//                 copies every request into the slot's device buffer and
+//                 writes the segment offsets, the segmented sort sorts them
+//                 all, k_serve_scatter copies the results back; the three
+//                 steps are captured once per slot into a hipGraph (fixed
+//                 capacity => fixed launch parameters) and replayed.
 //   completion    a second host thread waits for each slot's event in launch
 //                 order and stamps its requests' completion time.
 #include <hip/hip_runtime.h>
@@ -32,6 +37,7 @@
 #include <vector>
 
 #include "../../include/rsort_c.h"
+#include "rsort_launch.hpp"
 #include "rsort_tuning.hpp"
 
 namespace rsa {
@@ -177,7 +183,12 @@ struct Slot {
     uint8_t* host = nullptr;      // host-mapped: header | chunks | offsets
     uint8_t* host_dev = nullptr;  // its device address
     hipGraphExec_t exec = nullptr;
+    uint8_t* reqs = nullptr;      // host-mapped request list (direct form)
+    uint8_t* reqs_dev = nullptr;
+    void* flat_ws = nullptr;      // workspace of rsort_sort for long requests (direct form)
+    size_t flat_ws_bytes = 0;
     std::vector<uint64_t> ids;    // requests of the batch in flight
+    std::vector<SegReq> items;    // their buffers and sizes
     uint64_t batch = 0;
 };
 
@@ -187,6 +198,8 @@ struct Server {
     rsort_serve_config cfg{};
     int device = 0;
     uint32_t kb = 0, vb = 0, ve = 0, vw = 0;
+    bool direct = true;            // direct form (else batched)
+    uint32_t cap[kSegBins] = {};   // LDS bin capacities of the element type
     uint64_t max_chunks = 0;
     std::vector<Slot> slots;
     std::mutex mu;
@@ -225,6 +238,57 @@ struct Server {
                                                      cfg.batch_requests, sl.ws, &tb, sl.stream);
         if (st != RSORT_SUCCESS) return hipErrorLaunchFailure;
         return launch_serve_copy(c, kb, ve, false, sl.stream);
+    }
+
+    // Direct form: bin the slot's requests on the host, one launch per
+    // non-empty bin, rsort_sort for each request longer than the LDS capacity.
+    hipError_t enqueue_direct(Slot& sl) {
+        ReqJob j{};
+        j.kb = kb;
+        j.vb = vb;
+        j.tw = make_tw_masks(cfg.desc.key_kind, kb, cfg.desc.descending != 0);
+        j.reqs = sl.reqs_dev;
+        const size_t nr = sl.items.size();
+        std::vector<uint32_t> bin(nr);
+        uint32_t in_bin[kSegBins] = {};
+        for (size_t i = 0; i < nr; ++i) {
+            const uint64_t n = sl.items[i].n;
+            uint32_t b = kSegBins;  // longer than every bin (or nothing to do)
+            if (n >= 2)
+                for (uint32_t c = 0; c < (uint32_t)kSegBins; ++c)
+                    if (n <= cap[c]) {
+                        b = c;
+                        break;
+                    }
+            bin[i] = b;
+            if (b < (uint32_t)kSegBins) ++in_bin[b];
+        }
+        uint32_t at = 0;
+        for (int b = 0; b < kSegBins; ++b) {
+            j.first[b] = at;
+            j.count[b] = 0;
+            at += in_bin[b];
+        }
+        SegReq* list = (SegReq*)sl.reqs;  // host-mapped; the slot is idle
+        for (size_t i = 0; i < nr; ++i)
+            if (bin[i] < (uint32_t)kSegBins) list[j.first[bin[i]] + j.count[bin[i]]++] = sl.items[i];
+        hipError_t e = hipSuccess;
+        switch (kb) {
+            case 1: e = launch_req_kb1(j, sl.stream); break;
+            case 2: e = launch_req_kb2(j, sl.stream); break;
+            case 4: e = launch_req_kb4(j, sl.stream); break;
+            case 8: e = launch_req_kb8(j, sl.stream); break;
+            default: e = launch_req_kb16(j, sl.stream); break;
+        }
+        for (size_t i = 0; i < nr && e == hipSuccess; ++i) {
+            const SegReq& r = sl.items[i];
+            if (bin[i] < (uint32_t)kSegBins || r.n < 2) continue;
+            size_t tb = sl.flat_ws_bytes;
+            if (rsort_sort(&cfg.desc, (void*)(uintptr_t)r.keys, (void*)(uintptr_t)r.vals, sl.flat_ws, &tb, r.n,
+                           sl.stream) != RSORT_SUCCESS)
+                e = hipErrorLaunchFailure;
+        }
+        return e;
     }
 
     void fail_with(rsort_status st, const char* what, hipError_t e) {
@@ -277,30 +341,41 @@ struct Server {
             ++launching;
             sl.batch = ++batches;
             const auto now = Clock::now();
-            // copy list (host-mapped; the slot is idle, so nothing reads it)
-            ServeHeader* h = (ServeHeader*)sl.host;
-            ServeChunk* ch = (ServeChunk*)(sl.host + header_bytes());
-            uint64_t* off = (uint64_t*)(sl.host + header_bytes() + chunks_bytes());
-            uint64_t nch = 0, pos = 0;
+            // snapshot of the batch (the records are only read under the lock)
+            sl.items.resize(sl.ids.size());
             for (size_t i = 0; i < sl.ids.size(); ++i) {
                 Request& r = req[sl.ids[i]];
                 r.t_launch = now;
                 r.batch = sl.batch;
-                off[i] = pos;
-                for (uint64_t b = 0; b < r.n; b += kServeChunk) {
-                    ServeChunk& c = ch[nch++];
-                    c.keys = (uint64_t)(uintptr_t)r.keys + b * kb;
-                    c.vals = vb ? (uint64_t)(uintptr_t)r.vals + b * vb : 0;
-                    c.len = r.n - b < kServeChunk ? r.n - b : kServeChunk;
-                    c.offset = pos + b;
-                }
-                pos += r.n;
+                sl.items[i] = SegReq{(uint64_t)(uintptr_t)r.keys, (uint64_t)(uintptr_t)r.vals, r.n, 0};
             }
-            h->chunks = nch;
-            h->requests = sl.ids.size();
-            h->total = pos;
             lk.unlock();
-            hipError_t e = sl.exec ? hipGraphLaunch(sl.exec, sl.stream) : enqueue_batch(sl);
+            hipError_t e = hipSuccess;
+            if (direct) {
+                e = enqueue_direct(sl);
+            } else {
+                // copy list (host-mapped; the slot is idle, so nothing reads it)
+                ServeHeader* h = (ServeHeader*)sl.host;
+                ServeChunk* ch = (ServeChunk*)(sl.host + header_bytes());
+                uint64_t* off = (uint64_t*)(sl.host + header_bytes() + chunks_bytes());
+                uint64_t nch = 0, pos = 0;
+                for (size_t i = 0; i < sl.items.size(); ++i) {
+                    const SegReq& r = sl.items[i];
+                    off[i] = pos;
+                    for (uint64_t b0 = 0; b0 < r.n; b0 += kServeChunk) {
+                        ServeChunk& c = ch[nch++];
+                        c.keys = r.keys + b0 * kb;
+                        c.vals = vb ? r.vals + b0 * vb : 0;
+                        c.len = r.n - b0 < kServeChunk ? r.n - b0 : kServeChunk;
+                        c.offset = pos + b0;
+                    }
+                    pos += r.n;
+                }
+                h->chunks = nch;
+                h->requests = sl.items.size();
+                h->total = pos;
+                e = sl.exec ? hipGraphLaunch(sl.exec, sl.stream) : enqueue_batch(sl);
+            }
             if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
             lk.lock();
             --launching;
@@ -370,6 +445,8 @@ void destroy_slots(Server* s) {
         if (sl.offsets) (void)hipFree(sl.offsets);
         if (sl.ws) (void)hipFree(sl.ws);
         if (sl.host) (void)hipHostFree(sl.host);
+        if (sl.reqs) (void)hipHostFree(sl.reqs);
+        if (sl.flat_ws) (void)hipFree(sl.flat_ws);
     }
     s->slots.clear();
 }
@@ -396,6 +473,17 @@ rsort_status rsort_serve_create(const rsort_serve_config* cfg, rsort_server** ou
     s->ve = s->vb == 0 ? 1 : (s->vb % 16 == 0 ? 16 : s->vb % 8 == 0 ? 8 : s->vb % 4 == 0 ? 4 : s->vb % 2 == 0 ? 2 : 1);
     s->vw = s->vb / s->ve;
     s->max_chunks = cfg->batch_requests + cfg->batch_keys / rsa::kServeChunk + 1;
+    const bool pow2v = s->vb == 0 || s->vb == 1 || s->vb == 2 || s->vb == 4 || s->vb == 8 || s->vb == 16;
+    s->direct = !(cfg->flags & RSORT_SERVE_BATCHED) && pow2v;
+    for (int b = 0; b < rsa::kSegBins; ++b) s->cap[b] = rsa::seg_cap((int)s->kb, (int)s->vb, b);
+    size_t flat_need = 0;
+    if (s->direct) {
+        st = rsort_sort(&cfg->desc, nullptr, nullptr, nullptr, &flat_need, cfg->batch_keys, nullptr);
+        if (st != RSORT_SUCCESS) {
+            delete s;
+            return st;
+        }
+    }
     if (hipGetDevice(&s->device) != hipSuccess) {
         delete s;
         return RSORT_ERROR_DEVICE;
@@ -415,10 +503,17 @@ rsort_status rsort_serve_create(const rsort_serve_config* cfg, rsort_server** ou
             e = hipHostMalloc((void**)&sl.host, s->host_bytes(), hipHostMallocMapped | hipHostMallocCoherent);
         if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&sl.host_dev, sl.host, 0);
         if (e == hipSuccess) memset(sl.host, 0, s->host_bytes());
+        if (e == hipSuccess && s->direct) {
+            const size_t rb = (size_t)cfg->batch_requests * sizeof(rsa::SegReq);
+            e = hipHostMalloc((void**)&sl.reqs, rb, hipHostMallocMapped | hipHostMallocCoherent);
+            if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&sl.reqs_dev, sl.reqs, 0);
+            sl.flat_ws_bytes = flat_need;
+            if (e == hipSuccess) e = hipMalloc(&sl.flat_ws, flat_need);
+        }
     }
     // one direct run per slot (empty batch: also initialises the launchers'
     // one-time queries), then capture the batch into a graph
-    for (uint32_t i = 0; i < cfg->depth && e == hipSuccess; ++i) {
+    for (uint32_t i = 0; i < cfg->depth && e == hipSuccess && !s->direct; ++i) {
         auto& sl = s->slots[i];
         e = s->enqueue_batch(sl);
         if (e == hipSuccess) e = hipStreamSynchronize(sl.stream);
@@ -471,6 +566,38 @@ rsort_status rsort_serve_submit(rsort_server* srv, void* keys, void* values, uin
     return RSORT_SUCCESS;
 }
 
+rsort_status rsort_serve_submit_many(rsort_server* srv, void* const* keys, void* const* values, const uint64_t* n,
+                                     uint64_t count, uint64_t* request_ids) {
+    Server* s = (Server*)srv;
+    if (!s || (count && (!keys || !n || !request_ids))) return RSORT_ERROR_INVALID_ARGUMENT;
+    if (s->vb && count && !values) return RSORT_ERROR_INVALID_ARGUMENT;
+    for (uint64_t i = 0; i < count; ++i) {
+        if (n[i] > 0 && !keys[i]) return RSORT_ERROR_INVALID_ARGUMENT;
+        if (s->vb && n[i] > 0 && !values[i]) return RSORT_ERROR_INVALID_ARGUMENT;
+        if ((uintptr_t)keys[i] % s->kb || (s->vb && (uintptr_t)values[i] % s->ve)) return RSORT_ERROR_INVALID_ARGUMENT;
+        if (n[i] > s->cfg.batch_keys) return RSORT_ERROR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->error != RSORT_SUCCESS) return s->error;
+    if (s->stop) return RSORT_ERROR_BUSY;
+    if (s->cfg.max_pending && s->pending.size() + count > s->cfg.max_pending) return RSORT_ERROR_BUSY;
+    const auto now = rsa::Clock::now();
+    for (uint64_t i = 0; i < count; ++i) {
+        const uint64_t id = s->next_id++;
+        rsa::Request r{};
+        r.keys = keys[i];
+        r.vals = s->vb ? values[i] : nullptr;
+        r.n = n[i];
+        r.t_submit = now;
+        s->req.emplace(id, r);
+        s->pending.push_back(id);
+        s->pending_keys += n[i];
+        request_ids[i] = id;
+    }
+    s->cv_work.notify_one();
+    return RSORT_SUCCESS;
+}
+
 rsort_status rsort_serve_wait(rsort_server* srv, uint64_t request_id, uint64_t timeout_us, rsort_request_info* info) {
     Server* s = (Server*)srv;
     if (!s) return RSORT_ERROR_INVALID_ARGUMENT;
@@ -516,6 +643,7 @@ rsort_status rsort_serve_stats(rsort_server* srv, rsort_serve_stats_info* out) {
     out->pending = s->pending.size();
     out->in_flight = s->inflight.size();
     out->graphs = s->slots.empty() ? 0u : (s->slots[0].exec ? 1u : 0u);
+    out->direct = s->direct ? 1u : 0u;
     out->workspace_bytes = 0;
     for (auto& sl : s->slots)
         out->workspace_bytes += sl.ws_bytes + s->cfg.batch_keys * (s->kb + s->vb) + (s->cfg.batch_requests + 1) * 8u;

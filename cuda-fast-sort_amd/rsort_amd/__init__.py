@@ -99,10 +99,11 @@ class RequestInfo(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     _fields_ = [("batches", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("completed_keys", ctypes.c_uint64),
                 ("pending", ctypes.c_uint64), ("in_flight", ctypes.c_uint64), ("graphs", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32), ("workspace_bytes", ctypes.c_uint64)]
+                ("direct", ctypes.c_uint32), ("workspace_bytes", ctypes.c_uint64)]
 
 
 SERVE_NO_GRAPH = 1
+SERVE_BATCHED = 2
 
 _lib = None
 
@@ -131,10 +132,12 @@ def lib() -> ctypes.CDLL:
         L.rsort_serve_create.argtypes = [ctypes.POINTER(ServeConfig), ctypes.POINTER(vp)]
         L.rsort_serve_submit.argtypes = [vp, vp, vp, u64, ctypes.POINTER(ctypes.c_uint64)]
         L.rsort_serve_wait.argtypes = [vp, u64, u64, ctypes.POINTER(RequestInfo)]
+        L.rsort_serve_submit_many.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                              ctypes.POINTER(ctypes.c_uint64), u64, ctypes.POINTER(ctypes.c_uint64)]
         L.rsort_serve_flush.argtypes = [vp]
         L.rsort_serve_stats.argtypes = [vp, ctypes.POINTER(ServeStats)]
         L.rsort_serve_destroy.argtypes = [vp]
-        for f in ("create", "submit", "wait", "flush", "stats", "destroy"):
+        for f in ("create", "submit", "submit_many", "wait", "flush", "stats", "destroy"):
             getattr(L, "rsort_serve_" + f).restype = i32
         L.rsort_status_string.argtypes = [i32]
         L.rsort_status_string.restype = ctypes.c_char_p
@@ -296,12 +299,16 @@ def segmented_sort(keys, offsets, temp_bytes, d_workspace, values=None, descendi
 # Serving loop (no reference counterpart): continuous batching of requests
 # ---------------------------------------------------------------------------
 class Server:
-    """Batches sort requests (device tensors, sorted in place) into segmented
-    sorts on `depth` slots of `batch_keys` keys / `batch_requests` requests."""
+    """Batches sort requests (device tensors, sorted in place) on `depth`
+    slots of `batch_keys` keys / `batch_requests` requests.
+
+    mode: "direct" (LDS-bin launches straight on the request buffers),
+    "graph" (gather -> segmented sort -> scatter replayed from a hipGraph) or
+    "batched" (the same three steps launched directly)."""
 
     def __init__(self, key_type: KeyType, value_bytes: int = 0, descending: bool = False, batch_keys: int = 1 << 24,
-                 batch_requests: int = 4096, window_us: int = 200, depth: int = 2, max_pending: int = 0,
-                 graphs: bool = True):
+                 batch_requests: int = 4096, window_us: int = 0, depth: int = 2, max_pending: int = 0,
+                 mode: str = "direct"):
         cfg = ServeConfig()
         cfg.desc = make_desc(key_type.kind, key_type.nbytes, value_bytes, descending)
         cfg.batch_keys = batch_keys
@@ -309,7 +316,7 @@ class Server:
         cfg.window_us = window_us
         cfg.depth = depth
         cfg.max_pending = max_pending
-        cfg.flags = 0 if graphs else SERVE_NO_GRAPH
+        cfg.flags = {"direct": 0, "graph": SERVE_BATCHED, "batched": SERVE_BATCHED | SERVE_NO_GRAPH}[mode]
         h = ctypes.c_void_p()
         _check(lib().rsort_serve_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
@@ -320,6 +327,16 @@ class Server:
         n = int(keys.shape[0]) if n is None else int(n)
         _check(lib().rsort_serve_submit(self._h, _ptr(keys), _ptr(values), n, ctypes.byref(rid)))
         return int(rid.value)
+
+    def submit_many(self, key_ptrs, val_ptrs, sizes) -> list[int]:
+        """Submits requests given as device addresses and sizes in one call."""
+        count = len(sizes)
+        kp = (ctypes.c_void_p * count)(*key_ptrs)
+        vp_ = (ctypes.c_void_p * count)(*(val_ptrs if val_ptrs is not None else [None] * count))
+        nn = (ctypes.c_uint64 * count)(*sizes)
+        ids = (ctypes.c_uint64 * count)()
+        _check(lib().rsort_serve_submit_many(self._h, kp, vp_ if val_ptrs is not None else None, nn, count, ids))
+        return list(ids)
 
     def wait(self, request_id: int, timeout_us: int = 10_000_000) -> RequestInfo:
         info = RequestInfo()

@@ -149,14 +149,19 @@ rsort_status rsort_segmented_plan(const rsort_desc* desc, uint64_t n, uint32_t o
  * A request is a device array of n keys (+ values) of the configured type,
  * sorted in place like rsort_sort.  A scheduler thread closes a batch when a
  * slot is full (batch_keys / batch_requests), when its oldest request has
- * waited window_us, or on flush; the batch runs as gather -> segmented sort
- * -> scatter on the slot's stream (captured once into a hipGraph unless
- * RSORT_SERVE_NO_GRAPH).  Admission: a request must fit one slot
- * (UNSUPPORTED otherwise) and the queue holds at most max_pending requests
- * (BUSY).  All device memory (depth slots) is allocated at create.
- * The request's buffers must stay valid until rsort_serve_wait returns.
+ * waited window_us (0: as soon as a slot is free), or on flush.  Direct form
+ * (default, power-of-two value widths): requests are binned by size on the
+ * host and each LDS bin with requests is one launch sorting every request in
+ * place; longer requests get their own rsort_sort.  Batched form
+ * (RSORT_SERVE_BATCHED): gather -> segmented sort -> scatter through the
+ * slot's buffers, captured once into a hipGraph unless RSORT_SERVE_NO_GRAPH.
+ * Admission: a request must fit one slot (UNSUPPORTED otherwise) and the
+ * queue holds at most max_pending requests (BUSY).  All device memory (depth
+ * slots) is allocated at create.  The request's buffers must stay valid
+ * until rsort_serve_wait returns.
  */
 #define RSORT_SERVE_NO_GRAPH 1u
+#define RSORT_SERVE_BATCHED 2u
 
 typedef struct rsort_serve_config {
     rsort_desc desc;          /* element type and order of every request */
@@ -181,12 +186,15 @@ typedef struct rsort_request_info {
 typedef struct rsort_serve_stats {
     uint64_t batches, completed, completed_keys, pending, in_flight;
     uint32_t graphs;     /* 1 when batches replay captured graphs */
-    uint32_t reserved;
+    uint32_t direct;     /* 1 in the direct form */
     uint64_t workspace_bytes;
 } rsort_serve_stats_info;
 
 rsort_status rsort_serve_create(const rsort_serve_config* cfg, rsort_server** out);
 rsort_status rsort_serve_submit(rsort_server* s, void* keys, void* values, uint64_t n, uint64_t* request_id);
+/* Submits `count` requests at once (all or none; ids in request_ids). */
+rsort_status rsort_serve_submit_many(rsort_server* s, void* const* keys, void* const* values, const uint64_t* n,
+                                     uint64_t count, uint64_t* request_ids);
 /* Blocks until the request is done (BUSY after timeout_us; 0 = poll) and releases its record. */
 rsort_status rsort_serve_wait(rsort_server* s, uint64_t request_id, uint64_t timeout_us, rsort_request_info* info);
 /* Closes the open batch now and waits until nothing is queued or in flight. */

@@ -27,20 +27,21 @@ def _requests(orc, count, seed, max_len=5000, long_every=0):
     return keys
 
 
-@pytest.mark.parametrize("graphs", [True, False])
+@pytest.mark.parametrize("mode", ["direct", "graph", "batched"])
 @pytest.mark.parametrize("desc", [False, True])
-def test_alone_equals_mixed(rs, orc, graphs, desc):
-    print(f"[case] serve graphs={graphs} desc={desc}", flush=True)
+def test_alone_equals_mixed(rs, orc, mode, desc):
+    print(f"[case] serve mode={mode} desc={desc}", flush=True)
     host = _requests(orc, 300, seed=3 + int(desc), long_every=37)
     dk = [torch.from_numpy(k.copy()).cuda() for k in host]
     dv = [torch.arange(k.size, dtype=torch.int32, device="cuda") for k in host]
     torch.cuda.synchronize()
     with rs.Server(rs.KeyType(0, 4), value_bytes=4, descending=desc, batch_keys=1 << 20, batch_requests=64,
-                   window_us=100, depth=3, graphs=graphs) as srv:
+                   window_us=100, depth=3, mode=mode) as srv:
         ids = [srv.submit(k, v) for k, v in zip(dk, dv)]
         infos = [srv.wait(i) for i in ids]
         st = srv.stats()
-        assert st.completed == len(ids) and st.pending == 0 and st.graphs == int(graphs)
+        assert st.completed == len(ids) and st.pending == 0
+        assert st.graphs == int(mode == "graph") and st.direct == int(mode == "direct")
         assert st.batches >= 2  # 300 requests cannot share one 64-request slot
     for k, v, d_k, d_v, info in zip(host, [np.arange(k.size, dtype=np.int32) for k in host], dk, dv, infos):
         ek, ev = orc.sort(k, v, 0, 4, desc, vb=4)
@@ -65,11 +66,12 @@ def test_same_request_alone_and_in_mix(rs, orc):
     assert np.array_equal(a.cpu().numpy(), np.sort(probe))
 
 
-def test_key_kinds_and_values(rs, orc):
+@pytest.mark.parametrize("mode", ["direct", "graph"])
+def test_key_kinds_and_values(rs, orc, mode):
     from helpers import random_keys, random_values
 
-    for kind, kb, vb in ((2, 4, 8), (1, 8, 4), (0, 2, 0), (2, 8, 12)):
-        print(f"[case] serve kind={kind} kb={kb} vb={vb}", flush=True)
+    for kind, kb, vb in ((2, 4, 8), (1, 8, 4), (0, 2, 0), (2, 8, 12), (0, 16, 16), (1, 1, 2)):
+        print(f"[case] serve mode={mode} kind={kind} kb={kb} vb={vb}", flush=True)
         rng = np.random.default_rng(kb + vb)
         sizes = [int(x) for x in rng.integers(1, 20000, size=40)]
         host_k = [random_keys(kind, kb, m, seed=i) for i, m in enumerate(sizes)]
@@ -77,7 +79,7 @@ def test_key_kinds_and_values(rs, orc):
         dk = [torch.from_numpy(k.copy()).cuda() for k in host_k]
         dv = [torch.from_numpy(v.copy()).cuda() if v is not None else None for v in host_v]
         with rs.Server(rs.KeyType(kind, kb), value_bytes=vb, batch_keys=1 << 19, batch_requests=32,
-                       window_us=20, depth=2) as srv:
+                       window_us=20, depth=2, mode=mode) as srv:
             ids = [srv.submit(k, v, n=m) for k, v, m in zip(dk, dv, sizes)]
             for i in ids:
                 srv.wait(i)
@@ -116,3 +118,16 @@ def test_flush_empty_and_zero_length(rs):
         i0, i1 = srv.submit(z), srv.submit(one)
         assert srv.wait(i0).n == 0 and srv.wait(i1).n == 1
         assert one.item() == 5
+
+
+def test_submit_many(rs, orc):
+    host = _requests(orc, 120, seed=8, max_len=20000)
+    dk = [torch.from_numpy(k.copy()).cuda() for k in host]
+    dv = [torch.arange(k.size, dtype=torch.int32, device="cuda") for k in host]
+    with rs.Server(rs.KeyType(0, 4), value_bytes=4, batch_keys=1 << 20, batch_requests=64, depth=2) as srv:
+        ids = srv.submit_many([k.data_ptr() for k in dk], [v.data_ptr() for v in dv], [k.size for k in host])
+        for i in ids:
+            srv.wait(i)
+    for k, d_k, d_v in zip(host, dk, dv):
+        ek, ev = orc.sort(k, np.arange(k.size, dtype=np.int32), 0, 4, False, vb=4)
+        assert np.array_equal(d_k.cpu().numpy(), ek) and np.array_equal(d_v.cpu().numpy(), ev)
