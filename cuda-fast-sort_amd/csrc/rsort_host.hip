@@ -13,6 +13,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <dlfcn.h>
+
 #include <mutex>
 
 #include "../../include/rsort_c.h"
@@ -57,8 +59,100 @@ uint32_t pass_kernel_choice() {
     return v;
 }
 
+// ---------------------------------------------------------------------------
+// Tuning table: per (key width, value width) the pass kernel's tile
+// candidate, the histogram workgroups per CU and the segmented sort's LDS
+// threshold.  Built-in defaults, overridden by a text table ("key_bytes
+// value_bytes tile_candidate hist_blocks_per_cu lds_threshold" per line, '#'
+// comments) read once from $RSORT_AMD_TUNING_FILE or tuning_gfx950.txt next to
+// this library, and by rsort_tuning_set.  The table is written by the
+// autotuner (autotune.py).
+// ---------------------------------------------------------------------------
+struct TuneEntry {
+    uint32_t tile_cand, hist_bpc, lds_threshold;
+};
+
+int kb_index(uint32_t kb) { return kb == 1 ? 0 : kb == 2 ? 1 : kb == 4 ? 2 : kb == 8 ? 3 : kb == 16 ? 4 : -1; }
+int vb_index(uint32_t vb) {
+    return vb == 0 ? 0 : vb == 1 ? 1 : vb == 2 ? 2 : vb == 4 ? 3 : vb == 8 ? 4 : vb == 16 ? 5 : -1;
+}
+constexpr uint32_t kVbOf[6] = {0, 1, 2, 4, 8, 16};
+constexpr uint32_t kKbOf[5] = {1, 2, 4, 8, 16};
+
+std::mutex g_tune_mu;
+TuneEntry g_tune[5][6];
+bool g_tune_ready = false;
+
+TuneEntry tune_default(uint32_t kb, uint32_t vb) {
+    return TuneEntry{0u, hist_blocks_per_cu((int)kb), seg_cap((int)kb, (int)vb, kSegBins - 1)};
+}
+
+bool tune_valid(uint32_t kb, uint32_t vb, const TuneEntry& t) {
+    return t.tile_cand < (uint32_t)kTileCands && t.hist_bpc >= 1 && t.hist_bpc <= 8 &&
+           t.lds_threshold <= seg_cap((int)kb, (int)vb, kSegBins - 1);
+}
+
+// Reads a table file into g_tune (caller holds g_tune_mu); returns entries read or -1.
+int tune_read_file(const char* path) {
+    FILE* f = fopen(path, "r");
+    if (!f) return -1;
+    char line[256];
+    int n = 0;
+    while (fgets(line, sizeof(line), f)) {
+        char* h = strchr(line, '#');
+        if (h) *h = 0;
+        unsigned kb, vb, tc, bpc, thr;
+        if (sscanf(line, "%u %u %u %u %u", &kb, &vb, &tc, &bpc, &thr) != 5) continue;
+        const int ki = kb_index(kb), vi = vb_index(vb);
+        const TuneEntry t{tc, bpc, thr};
+        if (ki < 0 || vi < 0 || !tune_valid(kb, vb, t)) continue;
+        g_tune[ki][vi] = t;
+        ++n;
+    }
+    fclose(f);
+    return n;
+}
+
+void tune_reset_locked() {
+    for (int ki = 0; ki < 5; ++ki)
+        for (int vi = 0; vi < 6; ++vi) g_tune[ki][vi] = tune_default(kKbOf[ki], kVbOf[vi]);
+}
+
+// Defaults, then the persisted table ($RSORT_AMD_TUNING_FILE, else
+// tuning_gfx950.txt next to this library); returns entries read or -1.
+int tune_load_default_locked() {
+    tune_reset_locked();
+    g_tune_ready = true;
+    const char* env = getenv("RSORT_AMD_TUNING_FILE");
+    if (env && *env) return tune_read_file(env);
+    Dl_info di;
+    if (dladdr((const void*)&tune_default, &di) && di.dli_fname) {
+        char path[1024];
+        snprintf(path, sizeof(path), "%s", di.dli_fname);
+        char* slash = strrchr(path, '/');
+        if (slash) {
+            snprintf(slash + 1, sizeof(path) - (size_t)(slash + 1 - path), "tuning_gfx950.txt");
+            return tune_read_file(path);
+        }
+    }
+    return -1;
+}
+
+void tune_ensure_locked() {
+    if (!g_tune_ready) (void)tune_load_default_locked();
+}
+
+// Tuning of (kb, vb_core); value widths outside {0,1,2,4,8,16} use index mode (4-byte values).
+TuneEntry tune_get(uint32_t kb, uint32_t vb_core) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_ensure_locked();
+    const int ki = kb_index(kb), vi = vb_index(vb_core);
+    return (ki < 0 || vi < 0) ? tune_default(kb, vb_core) : g_tune[ki][vi];
+}
+
 struct Plan {
     uint32_t pass_kernel;
+    uint32_t tile_cand, hist_bpc;
     uint32_t kind, kb, vb_user, vb_core;
     bool desc, allow_skip, index_mode, wide;
     uint32_t passes;
@@ -98,11 +192,14 @@ rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
     p->wide = n > kNarrowMaxN;
     p->passes = p->kb;
     p->pass_kernel = pass_kernel_choice();
+    const TuneEntry te = tune_get(p->kb, p->vb_core);
+    p->tile_cand = tile_cands_built((int)p->kb, (int)p->vb_core) ? te.tile_cand : 0u;
+    p->hist_bpc = te.hist_bpc;
     if (p->pass_kernel == kPassChunk) {
         const ChunkCfg cc = chunk_cfg((int)p->kb, (int)p->vb_core);
         p->cfg = TileCfg{cc.threads, cc.ipt * cc.subs};
     } else {
-        p->cfg = tile_cfg((int)p->kb, (int)p->vb_core);
+        p->cfg = tile_cand((int)p->kb, (int)p->vb_core, (int)p->tile_cand);
     }
     p->tile = (uint64_t)p->cfg.threads * p->cfg.items;
     p->num_tiles = n ? (n + p->tile - 1) / p->tile : 0;
@@ -140,7 +237,7 @@ struct SegPlan {
     bool desc, index_mode, wide;
     uint32_t passes;
     uint64_t num_segments, chunks, max_large, max_tiles, tile;
-    uint32_t lds_cap;
+    uint32_t lds_cap;  // longest segment sorted in LDS (the largest bin within the tuned threshold)
     size_t off_tmp_keys, off_tmp_vals, off_idx, off_vals_copy, off_ctrl, off_chunk, off_queue, off_desc;
     size_t off_tile_seg, off_seg_gp, off_lb, total;
 };
@@ -169,7 +266,11 @@ rsort_status make_seg_plan(const rsort_desc* d, uint64_t n, uint32_t offset_byte
     p->chunks = (num_segments + kSegChunk - 1) / kSegChunk;
     const TileCfg c = tile_cfg((int)p->kb, (int)p->vb_core);
     p->tile = (uint64_t)c.threads * c.items;
-    p->lds_cap = seg_cap((int)p->kb, (int)p->vb_core, kSegBins - 1);
+    const uint32_t thr = tune_get(p->kb, p->vb_core).lds_threshold;
+    p->lds_cap = 0;
+    for (int b = 0; b < kSegBins; ++b)
+        if (seg_cap((int)p->kb, (int)p->vb_core, b) <= thr) p->lds_cap = seg_cap((int)p->kb, (int)p->vb_core, b);
+    if (p->lds_cap < 1) p->lds_cap = 1;  // no LDS bin: every segment of 2 or more keys is long
     p->max_large = num_segments ? n / ((uint64_t)p->lds_cap + 1u) : 0;
     if (p->max_large > num_segments) p->max_large = num_segments;
     p->max_tiles = p->max_large ? (n + p->tile - 1) / p->tile + p->max_large : 0;
@@ -266,6 +367,8 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
     uint8_t* base = (uint8_t*)(((uintptr_t)workspace + 255u) & ~(uintptr_t)255u);
     SortJob j{};
     j.pass_kernel = p.pass_kernel;
+    j.tile_cand = p.tile_cand;
+    j.hist_bpc = p.hist_bpc;
     j.kb = p.kb;
     j.vb = p.vb_core;
     j.wide = p.wide;
@@ -370,6 +473,7 @@ rsort_status run_segmented(const rsort_desc* d, void* keys, void* values, uint64
     j.chunks = (uint32_t)p.chunks;
     j.max_large = p.max_large;
     j.max_tiles = p.max_tiles;
+    j.lds_threshold = p.lds_cap;
     j.num_cus = cus;
     j.dbg = debug_slots();
     j.events = events;
@@ -506,6 +610,42 @@ rsort_status rsort_plan(const rsort_desc* desc, uint64_t n, rsort_plan_info* out
     out->workspace_bytes = p.total;
     out->lookback_bytes = p.lb_bytes;
     return RSORT_SUCCESS;
+}
+
+rsort_status rsort_tuning_get(uint32_t key_bytes, uint32_t value_bytes, rsort_tuning* out) {
+    if (!out) return rsa::fail(RSORT_ERROR_INVALID_ARGUMENT, "tuning output is NULL");
+    if (rsa::kb_index(key_bytes) < 0 || rsa::vb_index(value_bytes) < 0)
+        return rsa::fail(RSORT_ERROR_UNSUPPORTED, "no tuning entry for key %u / value %u bytes", key_bytes, value_bytes);
+    const rsa::TuneEntry t = rsa::tune_get(key_bytes, value_bytes);
+    out->tile_candidate = t.tile_cand;
+    out->hist_blocks_per_cu = t.hist_bpc;
+    out->lds_threshold = t.lds_threshold;
+    out->reserved = 0;
+    return RSORT_SUCCESS;
+}
+
+rsort_status rsort_tuning_set(uint32_t key_bytes, uint32_t value_bytes, const rsort_tuning* t) {
+    if (!t || t->reserved) return rsa::fail(RSORT_ERROR_INVALID_ARGUMENT, "tuning entry is NULL or reserved != 0");
+    const int ki = rsa::kb_index(key_bytes), vi = rsa::vb_index(value_bytes);
+    if (ki < 0 || vi < 0)
+        return rsa::fail(RSORT_ERROR_UNSUPPORTED, "no tuning entry for key %u / value %u bytes", key_bytes, value_bytes);
+    const rsa::TuneEntry e{t->tile_candidate, t->hist_blocks_per_cu, t->lds_threshold};
+    if (!rsa::tune_valid(key_bytes, value_bytes, e))
+        return rsa::fail(RSORT_ERROR_INVALID_ARGUMENT, "tuning out of range (tile candidate < %d, 1..8 histogram "
+                         "workgroups per CU, LDS threshold <= %u)", rsa::kTileCands,
+                         rsa::seg_cap((int)key_bytes, (int)value_bytes, rsa::kSegBins - 1));
+    std::lock_guard<std::mutex> lk(rsa::g_tune_mu);
+    rsa::tune_ensure_locked();
+    rsa::g_tune[ki][vi] = e;
+    return RSORT_SUCCESS;
+}
+
+int rsort_tuning_load(const char* path) {
+    std::lock_guard<std::mutex> lk(rsa::g_tune_mu);
+    if (!path) return rsa::tune_load_default_locked();
+    rsa::tune_reset_locked();
+    rsa::g_tune_ready = true;
+    return *path ? rsa::tune_read_file(path) : 0;
 }
 
 const char* rsort_status_string(rsort_status s) {

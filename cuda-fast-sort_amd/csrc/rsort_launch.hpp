@@ -15,6 +15,8 @@ namespace rsa {
 
 struct SortJob {
     uint32_t pass_kernel; // PassKernel
+    uint32_t tile_cand;   // tile_cand index of the pass kernel
+    uint32_t hist_bpc;    // histogram workgroups per CU
     uint32_t kb;          // key bytes 1,2,4,8,16
     uint32_t vb;          // value bytes as seen by the kernels: 0,1,2,4,8,16
     bool wide;            // 64-bit offsets and lookback words
@@ -65,6 +67,7 @@ struct SegJob {
     uint32_t chunks;      // ceil(num_segments / kSegChunk)
     uint64_t max_large;   // bound on long segments
     uint64_t max_tiles;   // bound on their tiles
+    uint32_t lds_threshold;   // longest segment sorted in LDS (bins above it are not used)
     int num_cus;
     unsigned long long* dbg;  // phase-timing slots (RSA_PHASE_TIMING builds)
     hipEvent_t* events;

@@ -26,7 +26,7 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     record_event(j, s);
 
     // 2. histogram of all passes, lookback clear, offsets, skip mask
-    const uint32_t hg = grid_for(j.n, hist_keys_per_block_iter(KB), j.num_cus, hist_blocks_per_cu(KB));
+    const uint32_t hg = grid_for(j.n, hist_keys_per_block_iter(KB), j.num_cus, j.hist_bpc ? j.hist_bpc : 1u);
     const uint32_t hgrid = hg < kHistMaxGrid ? hg : kHistMaxGrid;
     HistParams h{};
     h.keys = j.keys;
@@ -70,6 +70,14 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
             const uint32_t grid = j.num_tiles < (uint64_t)j.num_cus ? (uint32_t)j.num_tiles : (uint32_t)j.num_cus;
             hipLaunchKernelGGL((k_onesweep_p<KB, VB, WIDE, CC.threads, CC.ipt, CC.subs>), dim3(grid),
                                dim3(CC.threads), 0, s, pp);
+        } else if (tile_cands_built(KB, VB) && j.tile_cand == 1) {
+            constexpr TileCfg C1 = tile_cand(KB, VB, 1);
+            hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, C1.threads, C1.items>), dim3((uint32_t)j.num_tiles),
+                               dim3(C1.threads), 0, s, pp);
+        } else if (tile_cands_built(KB, VB) && j.tile_cand == 2) {
+            constexpr TileCfg C2 = tile_cand(KB, VB, 2);
+            hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, C2.threads, C2.items>), dim3((uint32_t)j.num_tiles),
+                               dim3(C2.threads), 0, s, pp);
         } else {
             hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, C.threads, C.items>), dim3((uint32_t)j.num_tiles),
                                dim3(C.threads), 0, s, pp);

@@ -93,7 +93,9 @@ hipError_t launch_seg_t(const SegJob& j, hipStream_t s) {
     a.off64 = j.off64;
     a.passes = P;
     a.tile = C.threads * C.items;
-    for (int b = 0; b < kSegBins; ++b) a.cap[b] = seg_cap(KB, VB, b);
+    // bins whose capacity exceeds the LDS threshold are not used: their
+    // segments go to the onesweep path
+    for (int b = 0; b < kSegBins; ++b) a.cap[b] = seg_cap(KB, VB, b) <= j.lds_threshold ? seg_cap(KB, VB, b) : 0u;
     a.tw = j.tw;
     a.dbg = j.dbg;
 

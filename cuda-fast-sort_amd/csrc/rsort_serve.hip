@@ -475,7 +475,12 @@ rsort_status rsort_serve_create(const rsort_serve_config* cfg, rsort_server** ou
     s->max_chunks = cfg->batch_requests + cfg->batch_keys / rsa::kServeChunk + 1;
     const bool pow2v = s->vb == 0 || s->vb == 1 || s->vb == 2 || s->vb == 4 || s->vb == 8 || s->vb == 16;
     s->direct = !(cfg->flags & RSORT_SERVE_BATCHED) && pow2v;
-    for (int b = 0; b < rsa::kSegBins; ++b) s->cap[b] = rsa::seg_cap((int)s->kb, (int)s->vb, b);
+    rsort_tuning tune{};
+    const uint32_t thr = rsort_tuning_get(s->kb, s->vb, &tune) == RSORT_SUCCESS ? tune.lds_threshold : 0u;
+    for (int b = 0; b < rsa::kSegBins; ++b) {
+        const uint32_t c = rsa::seg_cap((int)s->kb, (int)s->vb, b);
+        s->cap[b] = c <= thr ? c : 0u;  // bins above the tuned LDS threshold are not used
+    }
     size_t flat_need = 0;
     if (s->direct) {
         st = rsort_sort(&cfg->desc, nullptr, nullptr, nullptr, &flat_need, cfg->batch_keys, nullptr);

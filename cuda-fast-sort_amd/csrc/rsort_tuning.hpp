@@ -96,6 +96,23 @@ RSA_HD constexpr TileCfg tile_cfg(int key_bytes, int val_bytes) {
     return TileCfg{512, 6};
 }
 
+// Tile candidates of the pass kernel the autotuner chooses from.  Candidate
+// 0 is tile_cfg; 1 and 2 (fewer items per thread; half the threads) are
+// built for the widths sorted most (4- and 8-byte keys, keys only or with
+// values of the key's width) and fall back to candidate 0 elsewhere.
+constexpr int kTileCands = 3;
+
+RSA_HD constexpr bool tile_cands_built(int key_bytes, int val_bytes) {
+    return (key_bytes == 4 || key_bytes == 8) && (val_bytes == 0 || val_bytes == key_bytes);
+}
+
+RSA_HD constexpr TileCfg tile_cand(int key_bytes, int val_bytes, int cand) {
+    const TileCfg d = tile_cfg(key_bytes, val_bytes);
+    if (!tile_cands_built(key_bytes, val_bytes) || cand <= 0 || cand >= kTileCands) return d;
+    if (cand == 1) return TileCfg{d.threads, d.items - 4u};
+    return TileCfg{d.threads / 2u, d.items};
+}
+
 // ---------------------------------------------------------------------------
 // Segmented sort.  A segment of at most seg_cap(kb, vb, kSegBins-1) keys is
 // sorted by one workgroup entirely in LDS; bin b uses 64 << b threads holding

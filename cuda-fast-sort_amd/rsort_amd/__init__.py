@@ -102,6 +102,11 @@ class ServeStats(ctypes.Structure):
                 ("direct", ctypes.c_uint32), ("workspace_bytes", ctypes.c_uint64)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("tile_candidate", ctypes.c_uint32), ("hist_blocks_per_cu", ctypes.c_uint32),
+                ("lds_threshold", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
 SERVE_NO_GRAPH = 1
 SERVE_BATCHED = 2
 
@@ -137,6 +142,12 @@ def lib() -> ctypes.CDLL:
         L.rsort_serve_flush.argtypes = [vp]
         L.rsort_serve_stats.argtypes = [vp, ctypes.POINTER(ServeStats)]
         L.rsort_serve_destroy.argtypes = [vp]
+        L.rsort_tuning_get.argtypes = [u32, u32, ctypes.POINTER(Tuning)]
+        L.rsort_tuning_get.restype = i32
+        L.rsort_tuning_set.argtypes = [u32, u32, ctypes.POINTER(Tuning)]
+        L.rsort_tuning_set.restype = i32
+        L.rsort_tuning_load.argtypes = [ctypes.c_char_p]
+        L.rsort_tuning_load.restype = i32
         for f in ("create", "submit", "submit_many", "wait", "flush", "stats", "destroy"):
             getattr(L, "rsort_serve_" + f).restype = i32
         L.rsort_status_string.argtypes = [i32]
@@ -361,3 +372,24 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+# ---------------------------------------------------------------------------
+# Tuning table (what autotune.py chooses and persists)
+# ---------------------------------------------------------------------------
+def tuning_get(key_bytes: int, value_bytes: int) -> Tuning:
+    t = Tuning()
+    _check(lib().rsort_tuning_get(key_bytes, value_bytes, ctypes.byref(t)))
+    return t
+
+
+def tuning_set(key_bytes: int, value_bytes: int, tile_candidate: int, hist_blocks_per_cu: int,
+               lds_threshold: int) -> None:
+    t = Tuning(tile_candidate, hist_blocks_per_cu, lds_threshold, 0)
+    _check(lib().rsort_tuning_set(key_bytes, value_bytes, ctypes.byref(t)))
+
+
+def tuning_load(path: str | None) -> int:
+    """Reset to the built-in defaults, then read `path` (None: the persisted
+    table found at first use; "": built-in defaults only)."""
+    return int(lib().rsort_tuning_load(None if path is None else path.encode()))

@@ -142,6 +142,37 @@ rsort_status rsort_segmented_plan(const rsort_desc* desc, uint64_t n, uint32_t o
                                   rsort_segmented_plan_info* out);
 
 /*
+ * Tuning table (what the autotuner chooses), per key width x value width
+ * (value widths other than 0,1,2,4,8,16 use the 4-byte entry):
+ *   tile_candidate      pass-kernel tile: 0 = the built-in geometry, 1 = four
+ *                       items fewer per thread, 2 = half the threads (1 and 2
+ *                       exist for 4- and 8-byte keys, keys only or with values
+ *                       of the key's width; elsewhere they mean 0)
+ *   hist_blocks_per_cu  histogram workgroups per CU, 1..8
+ *   lds_threshold       segmented sort: longest segment sorted in LDS (LDS
+ *                       bins larger than it are not used), <= bin_capacity[4]
+ * Defaults are built in; at first use the library reads
+ * $RSORT_AMD_TUNING_FILE, else tuning_gfx950.txt next to the library (lines
+ * "key_bytes value_bytes tile_candidate hist_blocks_per_cu lds_threshold").
+ * Changing an entry changes workspace sizes: query again after a change.
+ * The reference fixes these per build in a table for sm_86
+ * (include/rsort/detail/tuning.cuh:138-229); there is no reference tuner.
+ */
+typedef struct rsort_tuning {
+    uint32_t tile_candidate;
+    uint32_t hist_blocks_per_cu;
+    uint32_t lds_threshold;
+    uint32_t reserved; /* must be 0 */
+} rsort_tuning;
+
+rsort_status rsort_tuning_get(uint32_t key_bytes, uint32_t value_bytes, rsort_tuning* out);
+rsort_status rsort_tuning_set(uint32_t key_bytes, uint32_t value_bytes, const rsort_tuning* t);
+/* Resets every entry to the built-in default, then reads `path`; NULL reads the
+ * persisted table found at first use, "" keeps the built-in defaults only.
+ * Returns the number of entries read, -1 if the file cannot be read. */
+int rsort_tuning_load(const char* path);
+
+/*
  * Serving loop: continuous batching of many small sort requests into
  * segmented sorts.  No counterpart in the reference (one sort per call); it
  * is for callers that sort many independent arrays as they arrive.

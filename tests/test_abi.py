@@ -175,3 +175,59 @@ def test_serve_rejects_bad_config_before_device_work(rs):
         h = ctypes.c_void_p()
         assert rs.lib().rsort_serve_create(ctypes.byref(cfg), ctypes.byref(h)) == 1, field
         assert not h.value
+
+
+# ---------------------------------------------------------------------------
+# Tuning table
+# ---------------------------------------------------------------------------
+def test_tuning_defaults_set_and_plan(rs, tmp_path):
+    rs.tuning_load("")  # built-in defaults only
+    t = rs.tuning_get(4, 4)
+    assert (t.tile_candidate, t.lds_threshold) == (0, 16384) and 1 <= t.hist_blocks_per_cu <= 8
+    d = rs.make_desc(0, 4, 4)
+    base = rs.plan(d, 1 << 20)
+    assert (base.threads, base.items_per_thread) == (512, 16)
+    try:
+        rs.tuning_set(4, 4, 1, 2, 8192)
+        p1 = rs.plan(d, 1 << 20)
+        assert (p1.threads, p1.items_per_thread) == (512, 12)
+        assert p1.num_tiles == ((1 << 20) + 6143) // 6144
+        rs.tuning_set(4, 4, 2, 2, 8192)
+        p2 = rs.plan(d, 1 << 20)
+        assert (p2.threads, p2.items_per_thread) == (256, 16)
+        # the LDS threshold moves segments of 8193..16384 keys to the onesweep path
+        sp = rs.segmented_plan(d, 1 << 22, 4, 1000)
+        assert sp.max_long_segments == (1 << 22) // 8193
+        # candidates are only built for the main widths: elsewhere they mean 0
+        rs.tuning_set(2, 0, 1, 1, 0)
+        assert rs.plan(rs.make_desc(0, 2, 0), 1000).items_per_thread == rs.plan(rs.make_desc(0, 2, 2), 1000).items_per_thread
+        # a table file round trip
+        f = tmp_path / "t.txt"
+        f.write_text("# kb vb tile hist lds\n4 4 1 3 4096\n8 0 2 2 8192\nbad line\n4 4 9 1 1\n")
+        assert rs.tuning_load(str(f)) == 2
+        t = rs.tuning_get(4, 4)
+        assert (t.tile_candidate, t.hist_blocks_per_cu, t.lds_threshold) == (1, 3, 4096)
+        assert rs.tuning_get(8, 0).tile_candidate == 2
+        assert rs.tuning_load(str(tmp_path / "missing.txt")) == -1
+    finally:
+        rs.tuning_load(None)
+    assert rs.plan(d, 1 << 20).items_per_thread == 16
+
+
+def test_persisted_table_is_read_at_first_use(rs):
+    """The autotuner's table next to the library is what the library uses."""
+    import os
+
+    path = os.path.join(os.path.dirname(rs.LIB_PATH), "tuning_gfx950.txt")
+    rows = [l.split() for l in open(path) if l.strip() and not l.startswith("#")]
+    assert rs.tuning_load(None) == len(rows) > 0
+    for kb, vb, tc, bpc, thr in rows:
+        t = rs.tuning_get(int(kb), int(vb))
+        assert (t.tile_candidate, t.hist_blocks_per_cu, t.lds_threshold) == (int(tc), int(bpc), int(thr))
+
+
+@pytest.mark.parametrize("args", [(4, 4, 3, 1, 0), (4, 4, 0, 0, 0), (4, 4, 0, 9, 0), (4, 4, 0, 1, 16385),
+                                  (3, 4, 0, 1, 0), (4, 12, 0, 1, 0)])
+def test_tuning_rejects_out_of_range(rs, args):
+    with pytest.raises(rs.RsortError):
+        rs.tuning_set(*args)

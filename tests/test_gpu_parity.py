@@ -211,3 +211,29 @@ def test_python_api_mirror(rs, orc):
     rs.sort(y, tb, torch.empty(tb, dtype=torch.uint8, device="cuda"), n)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(y.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("kb,vb", [(4, 0), (4, 4), (8, 0), (8, 8)])
+@pytest.mark.parametrize("cand", [1, 2])
+def test_tile_candidates(rs, orc, kb, vb, cand):
+    """Every pass-kernel tile candidate the autotuner may choose."""
+    try:
+        t = rs.tuning_get(kb, vb)
+        rs.tuning_set(kb, vb, cand, t.hist_blocks_per_cu, t.lds_threshold)
+        for n, desc in ((1000, False), (70001, True), (300007, False)):
+            keys = random_keys(0, kb, n, seed=cand * 100 + n)
+            vals = index_values(vb, n) if vb else None
+            check(rs, orc, keys, vals, 0, kb, vb, desc)
+    finally:
+        rs.tuning_load(None)
+
+
+@pytest.mark.parametrize("bpc", [1, 3])
+def test_histogram_blocks_per_cu(rs, orc, bpc):
+    try:
+        t = rs.tuning_get(4, 4)
+        rs.tuning_set(4, 4, t.tile_candidate, bpc, t.lds_threshold)
+        keys = random_keys(0, 4, 1_000_003, seed=bpc)
+        check(rs, orc, keys, index_values(4, 1_000_003), 0, 4, 4, False)
+    finally:
+        rs.tuning_load(None)

@@ -178,3 +178,18 @@ def test_python_mirror(rs, orc):
     ek, ev = orc.segmented_sort(ref_k, np.arange(n, dtype=np.int64), off.cpu().numpy(), 1, 4, True, vb=8)
     assert np.array_equal(keys.cpu().numpy(), ek)
     assert np.array_equal(vals.cpu().numpy(), ev)
+
+
+@pytest.mark.parametrize("threshold", [0, 1024, 4096])
+def test_lds_thresholds(rs, orc, threshold):
+    """Segments above the tuned LDS threshold take the onesweep path (0: all)."""
+    try:
+        t = rs.tuning_get(4, 4)
+        rs.tuning_set(4, 4, t.tile_candidate, t.hist_blocks_per_cu, threshold)
+        lengths = [0, 1, 2, 500, 1024, 1025, 3000, 4096, 4097, 9000, 20000]
+        n = int(sum(lengths))
+        keys = random_keys(0, 4, n, seed=threshold + 1)
+        vals = np.arange(n, dtype=np.uint32)
+        check(rs, orc, keys, vals, offsets_from_lengths(lengths), 0, 4, 4, threshold == 1024)
+    finally:
+        rs.tuning_load(None)
