@@ -142,6 +142,16 @@ def lib() -> ctypes.CDLL:
         L.rsort_serve_flush.argtypes = [vp]
         L.rsort_serve_stats.argtypes = [vp, ctypes.POINTER(ServeStats)]
         L.rsort_serve_destroy.argtypes = [vp]
+        L.rsort_init_keys.argtypes = [u32, u32, u32, vp, u64, vp]
+        L.rsort_init_keys.restype = i32
+        L.rsort_set_seed_radix.argtypes = [ctypes.c_int32]
+        L.rsort_set_seed_radix.restype = u32
+        L.rsort_check_sorted.argtypes = [ctypes.POINTER(Desc), vp, u64, vp, vp]
+        L.rsort_check_sorted.restype = i32
+        L.rsort_check_pairings.argtypes = [u32, u32, vp, vp, vp, u64, vp, vp]
+        L.rsort_check_pairings.restype = i32
+        L.rsort_digit_counts.argtypes = [ctypes.POINTER(Desc), vp, u64, vp, vp]
+        L.rsort_digit_counts.restype = i32
         L.rsort_tuning_get.argtypes = [u32, u32, ctypes.POINTER(Tuning)]
         L.rsort_tuning_get.restype = i32
         L.rsort_tuning_set.argtypes = [u32, u32, ctypes.POINTER(Tuning)]
@@ -393,3 +403,46 @@ def tuning_load(path: str | None) -> int:
     """Reset to the built-in defaults, then read `path` (None: the persisted
     table found at first use; "": built-in defaults only)."""
     return int(lib().rsort_tuning_load(None if path is None else path.encode()))
+
+
+# ---------------------------------------------------------------------------
+# Device generator and checks (reference utils.cuh:227-437 semantics)
+# ---------------------------------------------------------------------------
+def init_keys(keys, key_bytes: int, mode: int, seed: int, n: int | None = None, stream=None) -> None:
+    """Fill a device tensor with the reference generator's keys (mode 1 random, 2 byte_skip, 3 ascending)."""
+    n = keys.numel() * keys.element_size() // key_bytes if n is None else n
+    _check(lib().rsort_init_keys(key_bytes, mode, seed, _ptr(keys), n, _stream_ptr(stream)))
+
+
+def set_seed_radix(index: int) -> int:
+    return int(lib().rsort_set_seed_radix(index))
+
+
+def _counter():
+    import torch
+
+    return torch.zeros(1, dtype=torch.int64, device="cuda")
+
+
+def check_sorted(keys, desc: Desc, n: int, stream=None) -> int:
+    """Number of adjacent pairs out of order (0 = sorted)."""
+    c = _counter()
+    _check(lib().rsort_check_sorted(ctypes.byref(desc), _ptr(keys), n, c.data_ptr(), _stream_ptr(stream)))
+    return int(c.item())
+
+
+def check_pairings(keys_orig, keys_sorted, vals_sorted, key_bytes: int, value_bytes: int, n: int, stream=None) -> int:
+    """Number of positions violating the value = original index pairing or stability (0 = correct)."""
+    c = _counter()
+    _check(lib().rsort_check_pairings(key_bytes, value_bytes, _ptr(keys_orig), _ptr(keys_sorted), _ptr(vals_sorted),
+                                      n, c.data_ptr(), _stream_ptr(stream)))
+    return int(c.item())
+
+
+def digit_counts(keys, desc: Desc, n: int, stream=None):
+    """Per-pass digit counts of the twiddled keys as a [key_bytes, 256] int64 device tensor."""
+    import torch
+
+    out = torch.zeros(desc.key_bytes * 256, dtype=torch.int64, device="cuda")
+    _check(lib().rsort_digit_counts(ctypes.byref(desc), _ptr(keys), n, out.data_ptr(), _stream_ptr(stream)))
+    return out.view(desc.key_bytes, 256)

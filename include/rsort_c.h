@@ -142,6 +142,40 @@ rsort_status rsort_segmented_plan(const rsort_desc* desc, uint64_t n, uint32_t o
                                   rsort_segmented_plan_info* out);
 
 /*
+ * Device-side generator and checks (enqueued on `stream`; no host sync).
+ * Reference counterparts: init_keys / mix32 / set_seed_radix
+ * (include/rsort/detail/utils.cuh:227-317, 435-437), check_sorted (:341-365),
+ * check_pairings (:377-432), and the per-pass digit histogram
+ * (include/rsort/detail/histogram.cuh:88-211).  The checks ADD the number of
+ * violations to *d_violations (a device uint64 the caller zeroes), so a sort
+ * of any n is checked with one device word read back.
+ *
+ *   rsort_init_keys      key bit patterns of the reference's generator:
+ *                        mode 1 random (mix32 of index ^ seed), 2 byte_skip
+ *                        (every other byte 0x55 / 0x00), 3 ascending index
+ *   rsort_set_seed_radix the reference's seed of its i-th test array
+ *   rsort_check_sorted   adjacent pairs out of order (float / double compare
+ *                        as floating point, so NaNs never count; other kinds
+ *                        on their order-preserving word)
+ *   rsort_check_pairings values = original index (4 or 8 bytes): positions
+ *                        whose value is out of range, whose key differs from
+ *                        keys_orig[value], or whose equal-key successor has a
+ *                        smaller value (stability)
+ *   rsort_digit_counts   ADDS the per-pass digit counts of the (twiddled, per
+ *                        desc) keys to d_counts[p*256 + d], key_bytes * 256
+ *                        device uint64; equal before and after a sort
+ */
+rsort_status rsort_init_keys(uint32_t key_bytes, uint32_t mode, uint32_t seed, void* keys, uint64_t n, void* stream);
+uint32_t rsort_set_seed_radix(int32_t index);
+rsort_status rsort_check_sorted(const rsort_desc* desc, const void* keys, uint64_t n, unsigned long long* d_violations,
+                                void* stream);
+rsort_status rsort_check_pairings(uint32_t key_bytes, uint32_t value_bytes, const void* keys_orig,
+                                  const void* keys_sorted, const void* vals_sorted, uint64_t n,
+                                  unsigned long long* d_violations, void* stream);
+rsort_status rsort_digit_counts(const rsort_desc* desc, const void* keys, uint64_t n, unsigned long long* d_counts,
+                                void* stream);
+
+/*
  * Tuning table (what the autotuner chooses), per key width x value width
  * (value widths other than 0,1,2,4,8,16 use the 4-byte entry):
  *   tile_candidate      pass-kernel tile: 0 = the built-in geometry, 1 = four
