@@ -72,7 +72,24 @@ def build(jobs: int | None = None, verbose: bool = True, variant: str = "release
     os.replace(tmp, lib)
     if verbose:
         print(f"[build] {lib}", flush=True)
+    if variant == "release":
+        build_cli(verbose)
     return lib
+
+
+def build_cli(verbose: bool = True) -> str:
+    """cli/radix_gpu: the benchmark / validation driver, host C++ over the C ABI."""
+    exe = os.path.join(HERE, "cli", "radix_gpu")
+    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           os.path.join(HERE, "cli", "radix_gpu.cpp"), "-o", exe + ".tmp", "-L" + HERE, "-lrsort_amd",
+           "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"radix_gpu build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(exe + ".tmp", exe)
+    if verbose:
+        print(f"[build] {exe}", flush=True)
+    return exe
 
 
 if __name__ == "__main__":
