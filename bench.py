@@ -24,12 +24,24 @@ PEAK_HBM_GBS = 8000.0  # MI355X HBM3E peak, GB/s
 METRIC = "GKeys/s + % HBM roofline, 2^28 u32; batched-segment keys/s vs per-call"
 
 
+# Other workload configurations, each a bench line of its own:
+# name -> (description, key kind, key bytes, value bytes, log2 n)
+CONFIG_WORKLOADS = {
+    "plain20": ("sort u32 n=2^20, keys = raw mt19937(seed 0) outputs", 0, 4, 0, 20),
+    "u64low16": ("sort u64 n=2^27, low 16 bits uniform, upper 48 bits zero", 0, 8, 0, 27),
+    "zipf": ("sort u64 n=2^27, Zipf(s=1.1) over 2^20 ranks, rank -> fmix64 hash", 0, 8, 0, 27),
+    "f32": ("sort f32 n=2^27, N(0,1) with 1% +-0.0 and 0.1% +-inf", 2, 4, 0, 27),
+    "u128kv": ("sort_pairs u128/u64 n=2^26, hi uniform over 2^20 values, lo uniform over 2^64, values = index",
+               0, 16, 8, 26),
+}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["kv", "keys", "copy", "segments", "serve"], default="kv",
+    ap.add_argument("--workload", choices=["kv", "keys", "copy", "segments", "serve", *CONFIG_WORKLOADS], default="kv",
                     help="kv: sort_pairs u32/u32; keys: sort u32; copy: device copy of the keys "
                          "(stream-copy bandwidth, and the known-size kernel that calibrates HBM counters); "
                          "segments: one segmented sort of 100k lognormal-sized u32/u32 segments vs one "
@@ -98,6 +110,115 @@ def run_sort_bench(rs, n, with_values, descending, steps, warmup, seed=1):
     del ks, vs
     torch.cuda.empty_cache()
     return (t1 - t0) * 1e3 / steps, stages, ok, keys0, vals0
+
+
+def _fmix64(x):
+    """MurmurHash3's 64-bit finaliser on an int64 tensor (a bijection of 64-bit words)."""
+    m1 = 0xFF51AFD7ED558CCD - (1 << 64)
+    m2 = 0xC4CEB9FE1A85EC53 - (1 << 64)
+    for mul in (m1, m2, None):
+        x = x ^ ((x >> 33) & ((1 << 31) - 1))
+        if mul is not None:
+            x = x * mul
+    return x
+
+
+def gen_config(name, n, dev, seed):
+    """Keys (and values) of one workload configuration on the device: a uint8
+    tensor of n * key_bytes bytes, and an index tensor or None."""
+    import numpy as np
+    import torch
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    if name == "plain20":
+        keys = np.random.RandomState(0).randint(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32)
+        return torch.from_numpy(keys.view(np.uint8)).to(dev), None
+    if name == "u64low16":
+        k = torch.randint(0, 1 << 16, (n,), dtype=torch.int64, device=dev, generator=g)
+        return k.view(torch.uint8), None
+    if name == "zipf":
+        ranks = 1 << 20
+        w = torch.arange(1, ranks + 1, dtype=torch.float64, device=dev).pow(-1.1)
+        cdf = torch.cumsum(w, 0)
+        cdf /= cdf[-1].clone()
+        u = torch.rand(n, dtype=torch.float64, device=dev, generator=g)
+        r = torch.searchsorted(cdf, u, right=True).clamp_(max=ranks - 1)
+        return _fmix64(r + 1).view(torch.uint8), None
+    if name == "f32":
+        x = torch.randn(n, dtype=torch.float32, device=dev, generator=g)
+        u = torch.rand(n, dtype=torch.float32, device=dev, generator=g)
+        sign = torch.rand(n, dtype=torch.float32, device=dev, generator=g) < 0.5
+        x = torch.where(u < 0.011, torch.where(sign, -0.0, 0.0), x)
+        x = torch.where(u < 0.001, torch.where(sign, float("-inf"), float("inf")), x)
+        return x.view(torch.uint8), None
+    if name == "u128kv":
+        kw = torch.empty((n, 2), dtype=torch.int64, device=dev)
+        lo = torch.randint(-(2**63), 2**63 - 1, (n,), dtype=torch.int64, device=dev, generator=g)
+        kw[:, 0] = lo
+        kw[:, 1] = torch.randint(0, 1 << 20, (n,), dtype=torch.int64, device=dev, generator=g)
+        return kw.view(torch.uint8).reshape(-1), torch.arange(n, dtype=torch.int64, device=dev)
+    raise ValueError(name)
+
+
+def bench_config(rs, a):
+    """One of the other workload configurations: GKeys/s, the formula roofline
+    (every pass counted, also those the device skips), per-stage device times,
+    device-side checks (order, digit histograms, value pairing)."""
+    import torch
+
+    desc_s, kind, kb, vb, log2n = CONFIG_WORKLOADS[a.workload]
+    n = 1 << log2n
+    dev = torch.device("cuda:0")
+    keys0, vals0 = gen_config(a.workload, n, dev, 21)
+    d = rs.make_desc(kind, kb, vb, a.descending)
+    tb = rs.sort_raw(d, None, None, None, 0, n)
+    ws = torch.empty(tb, dtype=torch.uint8, device=dev)
+    before = rs.digit_counts(keys0, d, n)
+    total = a.steps + a.warmup
+    ks = [keys0.clone() for _ in range(total)]
+    vs = [vals0.clone() for _ in range(total)] if vb else [None] * total
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    for i in range(a.warmup):
+        rs.sort_raw(d, ks[i].data_ptr(), vs[i].data_ptr() if vb else None, ws.data_ptr(), tb, n, stream)
+    torch.cuda.synchronize()
+    stages = []
+    t0 = time.perf_counter()
+    for i in range(a.warmup, total):
+        stages.append(rs.sort_profiled(d, ks[i].data_ptr(), vs[i].data_ptr() if vb else None, ws.data_ptr(), tb, n,
+                                       stream))
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    ok = rs.check_sorted(ks[-1], d, n) == 0 and torch.equal(rs.digit_counts(ks[-1], d, n), before)
+    if vb:
+        ok = ok and rs.check_pairings(keys0, ks[-1], vs[-1], kb, vb, n) == 0
+    del ks, vs
+    dev_ms = sum(sum(s) for s in stages) / len(stages)
+    pass_ms = [sum(s[2 + p] for s in stages) / len(stages) for p in range(kb)]
+    longest = max(pass_ms)
+    executed = [p for p in range(kb) if pass_ms[p] > 0.05 * longest]
+    exec_avg = sum(pass_ms[p] for p in executed) / len(executed)
+    pass_bytes = 2.0 * n * (kb + vb)
+    formula_bytes = (1 + kb) * 2.0 * n * (kb + vb)
+    return {
+        "metric": f"GKeys/s, {desc_s}", "value": round(n / (dev_ms * 1e-3) / 1e9, 3), "unit": "GKeys/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dev_ms, 4),
+        "us_per_sort": round(dev_ms * 1e3, 2), "wall_ms_per_step": round(wall_ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": {4: "u32", 8: "u64", 16: "u128"}[kb] if kind != 2 else "f32",
+        "data": "synthetic",
+        "config": {"workload": a.workload, "description": desc_s, "n": n, "key_kind": kind, "key_bytes": kb,
+                   "value_bytes": vb, "order": "descending" if a.descending else "ascending"},
+        "roofline": {"bound": "hbm", "kernel": "k_onesweep (one executed LSD pass)",
+                     "achieved": round(pass_bytes / (exec_avg * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(pass_bytes / (exec_avg * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "traffic": None,
+                     "bytes_per_launch": pass_bytes, "avg_launch_ms": round(exec_avg, 5)},
+        "formula_roofline": {"bytes": formula_bytes, "passes_counted": kb, "device_ms": round(dev_ms, 4),
+                             "frac_device": round(formula_bytes / (dev_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
+        "executed_passes": executed,
+        "stages_ms_mean": [round(sum(s[i] for s in stages) / len(stages), 5) for i in range(len(stages[0]))],
+        "checked": bool(ok),
+    }
 
 
 def measured_traffic(kernel_prefix, n):
@@ -341,6 +462,9 @@ def main():
         return
     if a.workload == "serve":
         print(json.dumps(bench_serve(rs, a)), flush=True)
+        return
+    if a.workload in CONFIG_WORKLOADS:
+        print(json.dumps(bench_config(rs, a)), flush=True)
         return
     if a.workload == "copy":
         gbs, ms = copy_bandwidth(n, a.steps)

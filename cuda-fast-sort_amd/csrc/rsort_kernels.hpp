@@ -278,9 +278,9 @@ __device__ __forceinline__ void lds_st(T* p, T v) {
 // The LDS executes one wave's operations in order, so the sequence needs only
 // compiler fences (reference: ranker.cuh:62-170 match_early_counts, which
 // builds the peer mask from 8 ballots instead).
-template <int KB, int K>
+template <int KB, int K, typename CT>
 __device__ __forceinline__ void rank_wave(const typename KeyTraits<KB>::Reg (&key)[K], uint32_t sh, uint32_t wsel,
-                                          uint64_t* M, uint32_t* C, uint32_t lane, uint32_t (&dr)[K]) {
+                                          uint64_t* M, CT* C, uint32_t lane, uint32_t (&dr)[K]) {
     const uint64_t lanebit = 1ull << lane;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
@@ -291,7 +291,7 @@ __device__ __forceinline__ void rank_wave(const typename KeyTraits<KB>::Reg (&ke
             const uint32_t c = lds_ld(C + d0);
             rank = c + lane;
             RSA_COMPILER_FENCE();
-            if (lane == 0) lds_st(C + d0, c + 64u);
+            if (lane == 0) lds_st(C + d0, (CT)(c + 64u));
         } else {
             __hip_atomic_fetch_or(M + d, lanebit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
             RSA_COMPILER_FENCE();
@@ -303,7 +303,7 @@ __device__ __forceinline__ void rank_wave(const typename KeyTraits<KB>::Reg (&ke
             const uint32_t below = mask_below(m);
             rank = c + below;
             RSA_COMPILER_FENCE();
-            if (below == 0u) lds_st(C + d, c + (uint32_t)__builtin_popcountll(m));
+            if (below == 0u) lds_st(C + d, (CT)(c + (uint32_t)__builtin_popcountll(m)));
         }
         RSA_COMPILER_FENCE();
         dr[i] = (rank << 8) | d;
@@ -662,6 +662,16 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
 // ---------------------------------------------------------------------------
 // Onesweep pass.
 // ---------------------------------------------------------------------------
+// Minimum waves per SIMD the pass kernel is compiled for (keys only / with
+// values): 8 caps it at 64 VGPRs, which with the 16-bit digit counters lets
+// four 512-thread tiles share a CU.
+#ifndef RSA_PASS_MIN_WAVES_K
+#define RSA_PASS_MIN_WAVES_K 8
+#endif
+#ifndef RSA_PASS_MIN_WAVES_KV
+#define RSA_PASS_MIN_WAVES_KV 1
+#endif
+
 // Lookback windows: first round and maximum (words of one digit column).
 // Measured on MI355X at n = 2^28 (8K-key tiles): windows of 8 words beat
 // adaptive 4..16/32/64 -- every lookback word loaded costs bandwidth on the
@@ -702,8 +712,13 @@ struct OnesweepLds {
     static_assert(THREADS % 64 == 0 && THREADS >= 256, "workgroup must hold 256 digit owners");
     static_assert(TILE <= 65535u * 64u, "tile too large");
 
+    // Per-wave digit counters, then tile-local digit starts: both < TILE,
+    // so 16 bits suffice for tiles below 64K keys (4 KiB less LDS per
+    // 8-wave tile, which lets a fourth 8192-key tile fit in a CU's 160 KiB).
+    using Cnt = std::conditional_t<(TILE < 65536u), uint16_t, uint32_t>;
+
     __attribute__((aligned(16))) uint8_t big[BIG];
-    __attribute__((aligned(16))) uint32_t cnt[WAVES * 256];
+    __attribute__((aligned(16))) Cnt cnt[WAVES * 256];
     Off base[256];
     uint32_t wsum[4];
     uint32_t tile;
@@ -747,7 +762,8 @@ __device__ __forceinline__ uint32_t onesweep_begin(OnesweepLds<KB, VB, WIDE, THR
         uint4* m = (uint4*)(L.big + (size_t)w * 256 * sizeof(uint64_t));
         m[lane] = make_uint4(0u, 0u, 0u, 0u);
         m[lane + 64] = make_uint4(0u, 0u, 0u, 0u);
-        ((uint4*)(L.cnt + w * 256))[lane] = make_uint4(0u, 0u, 0u, 0u);
+        constexpr uint32_t cnt_units = 256u * sizeof(L.cnt[0]) / 16u;
+        if (lane < cnt_units) ((uint4*)(L.cnt + w * 256))[lane] = make_uint4(0u, 0u, 0u, 0u);
     }
     __syncthreads();
     return L.tile;
@@ -756,7 +772,7 @@ __device__ __forceinline__ uint32_t onesweep_begin(OnesweepLds<KB, VB, WIDE, THR
 // One LSD pass over the whole array: one workgroup per tile, tiles taken in
 // ticket order so every tile's predecessors are already running.
 template <int KB, int VB, bool WIDE, int THREADS, int K>
-__global__ __launch_bounds__(THREADS) void k_onesweep(PassParams a) {
+__global__ __launch_bounds__(THREADS, (VB == 0 ? RSA_PASS_MIN_WAVES_K : RSA_PASS_MIN_WAVES_KV)) void k_onesweep(PassParams a) {
     using KM = typename KeyTraits<KB>::Mem;
     using KR = typename KeyTraits<KB>::Reg;
     using VM = typename ValTraits<VB>::Mem;
