@@ -213,8 +213,14 @@ rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
         p->off_idx = reserve(&off, (size_t)n * 4u);
         p->off_vals_copy = reserve(&off, (size_t)n * p->vb_user);
     }
+    // control header, digit counts and lookback words are contiguous: one
+    // fill kernel clears them at the start of every sort
     p->off_ctrl = reserve(&off, sizeof(Ctrl));
-    p->off_counts = reserve(&off, (size_t)p->passes * kRadix * off_sz, 4);
+    p->off_counts = reserve(&off, (size_t)p->passes * kRadix * off_sz, 16);
+    p->lb_bytes = (size_t)p->num_tiles * kRadix * off_sz;
+    p->off_lb = reserve(&off, p->lb_bytes, 16);
+    if (p->off_counts != p->off_ctrl + sizeof(Ctrl) || p->off_lb != p->off_counts + (size_t)p->passes * kRadix * off_sz)
+        return fail(RSORT_ERROR_UNSUPPORTED, "workspace layout: cleared region is not contiguous");
     p->off_gp = reserve(&off, (size_t)p->passes * kRadix * off_sz, 4);
     {
         uint64_t g = (n + hist_keys_per_block_iter((int)p->kb) - 1) / hist_keys_per_block_iter((int)p->kb);
@@ -222,8 +228,6 @@ rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
         if (g < 1) g = 1;
         p->off_hist_partials = reserve(&off, (size_t)g * p->passes * kRadix * 4u);
     }
-    p->lb_bytes = (size_t)p->num_tiles * kRadix * off_sz;
-    p->off_lb = reserve(&off, p->lb_bytes);
     p->total = off + 256;  // slack so any 16-byte-aligned workspace pointer can be rounded up
     return RSORT_SUCCESS;
 }

@@ -160,9 +160,17 @@ __device__ __forceinline__ uint32_t key_digit(uint64_t x, uint32_t sh, uint32_t 
     const uint32_t w = wsel ? (uint32_t)(x >> 32) : (uint32_t)x;
     return __builtin_amdgcn_ubfe(w, sh, 8);
 }
+// The dword is picked by a mask merge, not a select: a select between the
+// two halves of a key held in a per-thread array is folded by the compiler
+// into a load from a computed address, which moves the whole key array to
+// scratch memory.  The masks pass through readfirstlane, which the optimiser
+// does not see through.
 __device__ __forceinline__ uint32_t key_digit(const U128& x, uint32_t sh, uint32_t wsel) {
-    const uint64_t h = (wsel & 2u) ? x.hi : x.lo;
-    const uint32_t w = (wsel & 1u) ? (uint32_t)(h >> 32) : (uint32_t)h;
+    const uint32_t mh = __builtin_amdgcn_readfirstlane(0u - ((wsel >> 1) & 1u));
+    const uint32_t mw = __builtin_amdgcn_readfirstlane(0u - (wsel & 1u));
+    const uint64_t m64 = ((uint64_t)mh << 32) | mh;
+    const uint64_t h = (x.hi & m64) | (x.lo & ~m64);
+    const uint32_t w = ((uint32_t)(h >> 32) & mw) | ((uint32_t)h & ~mw);
     return __builtin_amdgcn_ubfe(w, sh, 8);
 }
 
@@ -615,11 +623,19 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
         return;
     }
     Off* counts = (Off*)a.counts;
+    // all of a counter's rows are loaded at once (one coherent round trip,
+    // not one per row: at small n this fold is the kernel's critical path)
     for (uint32_t i = tid; i < (uint32_t)(P * 256); i += kHistThreads) {
+        const uint32_t* col = a.partials + (size_t)grp0 * (P * 256) + i;
+        uint32_t v[kHistGroup];
+#pragma unroll
+        for (int m = 0; m < kHistGroup; ++m)
+            v[m] = (uint32_t)m < gsize
+                       ? __hip_atomic_load(col + (size_t)m * (P * 256), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                       : 0u;
         uint32_t s = 0;
-        for (uint32_t m = 0; m < gsize; ++m)
-            s += __hip_atomic_load(a.partials + (size_t)(grp0 + m) * (P * 256) + i, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int m = 0; m < kHistGroup; ++m) s += v[m];
         if (s) atomicAdd(counts + i, (Off)s);
     }
     const uint32_t groups = (gridDim.x + kHistGroup - 1) / kHistGroup;
@@ -638,10 +654,17 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     // last workgroup: exclusive digit offsets per pass + uniform-pass mask
     // (counts are read with coherent atomic loads)
     Off* gp = (Off*)a.gp;
+    Off cp[P];  // every pass's count of digit tid, loaded at once
+    if (tid < 256) {
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+            cp[p] = __hip_atomic_load(counts + p * 256 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
     for (int p = 0; p < P; ++p) {
         Off c = 0, incl = 0;
         if (tid < 256) {
-            c = __hip_atomic_load(counts + p * 256 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            c = cp[p];
             if ((uint64_t)c == a.n) atomicOr(&s_skip, 1u << p);
             incl = wave_incl_scan<Off>(c);
             if (lane == 63) s_wsum[w] = incl;
@@ -662,26 +685,13 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
 // ---------------------------------------------------------------------------
 // Onesweep pass.
 // ---------------------------------------------------------------------------
-// Minimum waves per SIMD the pass kernel is compiled for (keys only / with
-// values): 8 caps it at 64 VGPRs, which with the 16-bit digit counters lets
-// four 512-thread tiles share a CU.
-#ifndef RSA_PASS_MIN_WAVES_K
-#define RSA_PASS_MIN_WAVES_K 8
-#endif
-#ifndef RSA_PASS_MIN_WAVES_KV
-#define RSA_PASS_MIN_WAVES_KV 1
-#endif
-
-// Lookback windows: first round and maximum (words of one digit column).
-// Measured on MI355X at n = 2^28 (8K-key tiles): windows of 8 words beat
-// adaptive 4..16/32/64 -- every lookback word loaded costs bandwidth on the
-// coherent path, and walks are long enough that 8 is rarely wasted.
 #ifndef RSA_LB_W0
 #define RSA_LB_W0 8
 #endif
 #ifndef RSA_LB_WMAX
 #define RSA_LB_WMAX 8
 #endif
+
 
 // Lookback words are read and written with agent-scope relaxed atomics
 // (coherent across the XCD L2s without cache maintenance); no other data
@@ -772,13 +782,15 @@ __device__ __forceinline__ uint32_t onesweep_begin(OnesweepLds<KB, VB, WIDE, THR
 // One LSD pass over the whole array: one workgroup per tile, tiles taken in
 // ticket order so every tile's predecessors are already running.
 template <int KB, int VB, bool WIDE, int THREADS, int K>
-__global__ __launch_bounds__(THREADS, (VB == 0 ? RSA_PASS_MIN_WAVES_K : RSA_PASS_MIN_WAVES_KV)) void k_onesweep(PassParams a) {
+__global__ __launch_bounds__(THREADS, pass_min_waves(KB, VB, K)) void k_onesweep(PassParams a) {
     using KM = typename KeyTraits<KB>::Mem;
     using KR = typename KeyTraits<KB>::Reg;
     using VM = typename ValTraits<VB>::Mem;
     using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
     constexpr int WAVES = THREADS / 64;
     constexpr uint32_t TILE = (uint32_t)THREADS * K;
+    constexpr int LB_W0 = RSA_LB_W0;
+    constexpr int LB_WMAX = RSA_LB_WMAX;
     __shared__ OnesweepLds<KB, VB, WIDE, THREADS, K> L;
 
     const uint32_t passes = a.passes;

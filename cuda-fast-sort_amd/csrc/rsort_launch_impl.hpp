@@ -16,13 +16,12 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     constexpr uint32_t P = KB;
 
     record_event(j, s);
-    // 1. control header + global digit counts
-    const uint32_t zero_words = (uint32_t)((sizeof(Ctrl) + (size_t)P * kRadix * sizeof(Off)) / 4u);
-    hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(zero_words, 256, j.num_cus, 1)), dim3(256), 0, s,
-                       (uint32_t*)j.ctrl, zero_words);
-    // lookback words start at flag 0 (never ready) for every epoch
-    hipLaunchKernelGGL(k_fill_zero16, dim3(grid_for(j.lb_bytes / 16u, 256u * 8u, j.num_cus, 4)), dim3(256), 0, s,
-                       (uint4*)j.lookback, (uint64_t)(j.lb_bytes / 16u));
+    // 1. control header, global digit counts and lookback words (flag 0: never
+    //    ready in any epoch), contiguous in the workspace: one fill
+    const uint64_t clear_units =
+        (uint64_t)((sizeof(Ctrl) + (size_t)P * kRadix * sizeof(Off) + j.lb_bytes) / 16u);
+    hipLaunchKernelGGL(k_fill_zero16, dim3(grid_for(clear_units, 256u * 8u, j.num_cus, 4)), dim3(256), 0, s,
+                       (uint4*)j.ctrl, clear_units);
     record_event(j, s);
 
     // 2. histogram of all passes, lookback clear, offsets, skip mask

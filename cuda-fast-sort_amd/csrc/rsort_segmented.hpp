@@ -467,6 +467,8 @@ __global__ __launch_bounds__(THREADS) void k_seg_onesweep(SegParams a) {
     using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
     constexpr int WAVES = THREADS / 64;
     constexpr uint32_t TILE = (uint32_t)THREADS * K;
+    constexpr int LB_W0 = RSA_LB_W0;
+    constexpr int LB_WMAX = RSA_LB_WMAX;
     __shared__ OnesweepLds<KB, VB, WIDE, THREADS, K> L;
 
     const uint32_t pass = a.pass;

@@ -64,36 +64,70 @@ RSA_HD constexpr ChunkCfg chunk_cfg(int key_bytes, int val_bytes) {
     return ChunkCfg{1024u, sub / 1024u, 3u};
 }
 
-#ifndef RSA_THREADS_4_0
-#define RSA_THREADS_4_0 512
+// Tile of the pass kernel per key / value width: threads x items per thread.
+// Measured on MI355X: fewer, larger tiles win -- every tile adds a column of
+// 256 lookback words that its successors must read -- as long as the
+// workgroups resident per CU still cover the latency.  Whole-call GKeys/s
+// against the earlier 512-thread tiles: u32 keys 2^28 1024x16 (two
+// workgroups per CU) 98 vs 82; u32/u32 1024x20 (one per CU) 56 vs 45; u64
+// 2^27 1024x8 28.8 vs 24.9; u64/u64 1024x10 15.8 vs 12.7; u128 2^26 1024x8
+// 5.9 vs 5.6; u128/u64 1024x6 4.45 vs 3.61.
+#ifndef RSA_TILE_K4_V0
+#define RSA_TILE_K4_V0 1024, 16
 #endif
-#ifndef RSA_ITEMS_4_0
-#define RSA_ITEMS_4_0 16
+#ifndef RSA_TILE_K4_V4
+#define RSA_TILE_K4_V4 1024, 20
 #endif
-#ifndef RSA_THREADS_4_4
-#define RSA_THREADS_4_4 512
+#ifndef RSA_TILE_K4_V8
+#define RSA_TILE_K4_V8 1024, 12
 #endif
-#ifndef RSA_ITEMS_4_4
-#define RSA_ITEMS_4_4 16
+#ifndef RSA_TILE_K4_V16
+#define RSA_TILE_K4_V16 1024, 8
 #endif
+#ifndef RSA_TILE_K8_V0
+#define RSA_TILE_K8_V0 1024, 8
+#endif
+#ifndef RSA_TILE_K8_V8
+#define RSA_TILE_K8_V8 1024, 10
+#endif
+#ifndef RSA_TILE_K8_V16
+#define RSA_TILE_K8_V16 1024, 8
+#endif
+#ifndef RSA_TILE_K16_V0
+#define RSA_TILE_K16_V0 1024, 8
+#endif
+#ifndef RSA_TILE_K16_VX
+#define RSA_TILE_K16_VX 1024, 6
+#endif
+#define RSA_TILE_CFG(...) TileCfg{__VA_ARGS__}
 
 // key_bytes in {1,2,4,8,16}; val_bytes in {0,1,2,4,8,16}
 RSA_HD constexpr TileCfg tile_cfg(int key_bytes, int val_bytes) {
     const int wide = key_bytes > val_bytes ? key_bytes : val_bytes;
     if (key_bytes <= 4) {
-        if (val_bytes == 0) return TileCfg{RSA_THREADS_4_0, RSA_ITEMS_4_0};
-        if (wide <= 4) return TileCfg{RSA_THREADS_4_4, RSA_ITEMS_4_4};
-        if (wide == 8) return TileCfg{512, 12};
-        return TileCfg{512, 8};
+        if (val_bytes == 0) return RSA_TILE_CFG(RSA_TILE_K4_V0);
+        if (wide <= 4) return RSA_TILE_CFG(RSA_TILE_K4_V4);
+        if (wide == 8) return RSA_TILE_CFG(RSA_TILE_K4_V8);
+        return RSA_TILE_CFG(RSA_TILE_K4_V16);
     }
     if (key_bytes == 8) {
-        if (val_bytes == 0) return TileCfg{512, 12};
-        if (wide == 8) return TileCfg{512, 10};
-        return TileCfg{512, 8};
+        if (val_bytes == 0) return RSA_TILE_CFG(RSA_TILE_K8_V0);
+        if (wide == 8) return RSA_TILE_CFG(RSA_TILE_K8_V8);
+        return RSA_TILE_CFG(RSA_TILE_K8_V16);
     }
     // 16-byte keys
-    if (val_bytes == 0) return TileCfg{512, 8};
-    return TileCfg{512, 6};
+    if (val_bytes == 0) return RSA_TILE_CFG(RSA_TILE_K16_V0);
+    return RSA_TILE_CFG(RSA_TILE_K16_VX);
+}
+
+// Minimum waves per SIMD the pass kernel is compiled for: 8 (at most 64
+// VGPRs) where the tile fits that without spilling, so that two 1024-thread
+// (or four 512-thread) workgroups share a CU; 1 (no bound) elsewhere.
+RSA_HD constexpr int pass_min_waves(int key_bytes, int val_bytes, int items) {
+    return (key_bytes <= 4 && (val_bytes == 0 || val_bytes == 4) && items <= 16) ||
+                   (key_bytes == 8 && (val_bytes == 0 || val_bytes == 8) && items <= 8)
+               ? 8
+               : 1;
 }
 
 // Tile candidates of the pass kernel the autotuner chooses from.  Candidate

@@ -186,21 +186,22 @@ def test_tuning_defaults_set_and_plan(rs, tmp_path):
     assert (t.tile_candidate, t.lds_threshold) == (0, 16384) and 1 <= t.hist_blocks_per_cu <= 8
     d = rs.make_desc(0, 4, 4)
     base = rs.plan(d, 1 << 20)
-    assert (base.threads, base.items_per_thread) == (512, 16)
+    assert (base.threads, base.items_per_thread) == (1024, 20)
     try:
         rs.tuning_set(4, 4, 1, 2, 8192)
         p1 = rs.plan(d, 1 << 20)
-        assert (p1.threads, p1.items_per_thread) == (512, 12)
-        assert p1.num_tiles == ((1 << 20) + 6143) // 6144
+        assert (p1.threads, p1.items_per_thread) == (1024, 16)
+        assert p1.num_tiles == ((1 << 20) + 16383) // 16384
         rs.tuning_set(4, 4, 2, 2, 8192)
         p2 = rs.plan(d, 1 << 20)
-        assert (p2.threads, p2.items_per_thread) == (256, 16)
+        assert (p2.threads, p2.items_per_thread) == (512, 20)
         # the LDS threshold moves segments of 8193..16384 keys to the onesweep path
         sp = rs.segmented_plan(d, 1 << 22, 4, 1000)
         assert sp.max_long_segments == (1 << 22) // 8193
         # candidates are only built for the main widths: elsewhere they mean 0
         rs.tuning_set(2, 0, 1, 1, 0)
-        assert rs.plan(rs.make_desc(0, 2, 0), 1000).items_per_thread == rs.plan(rs.make_desc(0, 2, 2), 1000).items_per_thread
+        assert (rs.plan(rs.make_desc(0, 2, 0), 1000).threads, rs.plan(rs.make_desc(0, 2, 0), 1000).items_per_thread) == (
+            1024, 16)
         # a table file round trip
         f = tmp_path / "t.txt"
         f.write_text("# kb vb tile hist lds\n4 4 1 3 4096\n8 0 2 2 8192\nbad line\n4 4 9 1 1\n")
@@ -211,7 +212,7 @@ def test_tuning_defaults_set_and_plan(rs, tmp_path):
         assert rs.tuning_load(str(tmp_path / "missing.txt")) == -1
     finally:
         rs.tuning_load(None)
-    assert rs.plan(d, 1 << 20).items_per_thread == 16
+    assert rs.plan(d, 1 << 20).items_per_thread == 20
 
 
 def test_persisted_table_is_read_at_first_use(rs):
