@@ -101,7 +101,7 @@ def run_sort_bench(rs, n, with_values, descending, steps, warmup, seed=1):
     t1 = time.perf_counter()
     # spot check of the last result (sortedness + pairing) on the device
     k = ks[-1].view(torch.int32) ^ (-(2**31))  # signed order == unsigned order of the bits
-    ok = bool((k[1:] >= k[:-1]).all().item())
+    ok = bool((k[1:] <= k[:-1]).all().item()) if descending else bool((k[1:] >= k[:-1]).all().item())
     if with_values:
         idx = vs[-1].long()
         ok = ok and bool((keys0[idx] == ks[-1]).all().item())

@@ -7,11 +7,15 @@ For each tuned (key bytes, value bytes) it times, with the library's own
 per-stage device events (rsort_sort_profiled):
   * every pass-kernel tile candidate            -> fastest mean pass time
   * histogram workgroups per CU in 1..4         -> fastest histogram
+and, with device events around the whole rsort_sort call at 2^20, 2^22,
+2^24 and 2^26 keys:
+  * every tile candidate again                  -> the small-sort candidate
+    and the size below which it is used (small_tile_candidate, small_log2n)
 and, with device events around rsort_segmented_sort on a batch of
 lognormal-sized segments (median 2048, sigma 0.7356, clipped [32, 2^22]):
   * the LDS threshold (each LDS bin capacity)   -> fastest segmented sort.
 The table (one line per entry: key_bytes value_bytes tile_candidate
-hist_blocks_per_cu lds_threshold) goes to --out, by default
+hist_blocks_per_cu lds_threshold small_tile_candidate small_log2n) goes to --out, by default
 tuning_gfx950.txt next to librsort_amd.so, and is printed.  Measurements are
 on synthetic uniform keys; the reference has no tuner (its geometry is a
 fixed sm_86 table, include/rsort/detail/tuning.cuh:138-229).
@@ -58,6 +62,48 @@ def time_flat(rs, torch, kb, vb, n, reps):
             passes.extend(st[2:2 + kb])
     del ws, keys0, vals0
     return sum(passes) / len(passes), sum(hist) / len(hist)
+
+
+def time_whole(rs, torch, kb, vb, n, reps):
+    """Median ms of whole rsort_sort calls (events around the call) for n keys."""
+    dev = torch.device("cuda:0")
+    keys0 = torch.randint(0, 256, (n * kb,), dtype=torch.uint8, device=dev)
+    vals0 = torch.randint(0, 256, (n * vb,), dtype=torch.uint8, device=dev) if vb else None
+    d = rs.make_desc(rs.KEY_UNSIGNED, kb, vb)
+    tb = rs.sort_raw(d, None, None, None, 0, n)
+    ws = torch.empty(tb, dtype=torch.uint8, device=dev)
+    k = keys0.clone()
+    v = vals0.clone() if vb else None
+    ms = []
+    for r in range(reps + 3):
+        k.copy_(keys0)
+        if vb:
+            v.copy_(vals0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rs.sort_raw(d, k.data_ptr(), v.data_ptr() if vb else None, ws.data_ptr(), tb, n,
+                    torch.cuda.current_stream())
+        e1.record()
+        torch.cuda.synchronize()
+        if r >= 3:
+            ms.append(e0.elapsed_time(e1))
+    ms.sort()
+    return ms[len(ms) // 2]
+
+
+def small_choice(best_by_size, large_cand):
+    """(small candidate, small_log2n) from the best candidate per measured
+    log2 size: the sizes from the smallest up whose best differs from the
+    large-sort candidate, up to the first that agrees."""
+    sizes = sorted(best_by_size)
+    below = []
+    for s in sizes:
+        if best_by_size[s] == large_cand:
+            break
+        below.append(s)
+    if not below:
+        return 0, 0
+    return best_by_size[below[-1]], below[-1] + 1
 
 
 def time_segmented(rs, torch, kb, vb, sizes, reps):
@@ -125,18 +171,30 @@ def main():
             rs.tuning_set(kb, vb, best_tile, best_bpc, cap)
             seg_ms[cap] = time_segmented(rs, torch, kb, vb, sizes, a.reps)
         best_cap = min(seg_ms, key=seg_ms.get)
-        rs.tuning_set(kb, vb, best_tile, best_bpc, best_cap)
-        lines.append(f"{kb} {vb} {best_tile} {best_bpc} {best_cap}")
+        # tile candidate of small sorts
+        small_ms, best_by_size = {}, {}
+        for l2 in [s for s in (20, 22, 24, 26) if s < a.log2n]:
+            row = {}
+            for c in range(3):
+                rs.tuning_set(kb, vb, c, best_bpc, best_cap)
+                row[c] = time_whole(rs, torch, kb, vb, 1 << l2, a.reps + 4)
+            small_ms[l2] = row
+            best_by_size[l2] = min(row, key=row.get)
+        small_cand, small_log2n = small_choice(best_by_size, best_tile)
+        rs.tuning_set(kb, vb, best_tile, best_bpc, best_cap, small_cand, small_log2n)
+        lines.append(f"{kb} {vb} {best_tile} {best_bpc} {best_cap} {small_cand} {small_log2n}")
         report.append({"key_bytes": kb, "value_bytes": vb, "pass_ms_by_tile_candidate": tile_ms,
                        "histogram_ms_by_blocks_per_cu": hist_ms, "segmented_ms_by_lds_threshold": seg_ms,
+                       "sort_ms_by_log2n_and_candidate": small_ms,
                        "chosen": {"tile_candidate": best_tile, "hist_blocks_per_cu": best_bpc,
-                                  "lds_threshold": best_cap}})
+                                  "lds_threshold": best_cap, "small_tile_candidate": small_cand,
+                                  "small_log2n": small_log2n}})
         print(json.dumps(report[-1]), flush=True)
     props = torch.cuda.get_device_properties(0)
     header = [f"# rsort tuning table for {getattr(props, 'gcnArchName', 'gfx950')} ({props.name}), written by autotune.py",
               f"# {time.strftime('%Y-%m-%d')}: flat sorts of 2^{a.log2n} uniform keys, segmented sorts of "
               f"{a.segments} lognormal segments, mean of {a.reps} runs each",
-              "# key_bytes value_bytes tile_candidate hist_blocks_per_cu lds_threshold"]
+              "# key_bytes value_bytes tile_candidate hist_blocks_per_cu lds_threshold small_tile_candidate small_log2n"]
     text = "\n".join(header + lines) + "\n"
     with open(a.out, "w") as f:
         f.write(text)

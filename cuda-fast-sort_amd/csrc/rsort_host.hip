@@ -62,14 +62,16 @@ uint32_t pass_kernel_choice() {
 // ---------------------------------------------------------------------------
 // Tuning table: per (key width, value width) the pass kernel's tile
 // candidate, the histogram workgroups per CU and the segmented sort's LDS
-// threshold.  Built-in defaults, overridden by a text table ("key_bytes
-// value_bytes tile_candidate hist_blocks_per_cu lds_threshold" per line, '#'
-// comments) read once from $RSORT_AMD_TUNING_FILE or tuning_gfx950.txt next to
+// threshold, plus the tile candidate of small sorts.  Built-in defaults,
+// overridden by a text table ("key_bytes value_bytes tile_candidate
+// hist_blocks_per_cu lds_threshold [small_tile_candidate small_log2n]" per
+// line, '#' comments) read once from $RSORT_AMD_TUNING_FILE or tuning_gfx950.txt next to
 // this library, and by rsort_tuning_set.  The table is written by the
 // autotuner (autotune.py).
 // ---------------------------------------------------------------------------
 struct TuneEntry {
     uint32_t tile_cand, hist_bpc, lds_threshold;
+    uint32_t small_tile_cand, small_log2n;  // candidate of sorts below 2^small_log2n keys
 };
 
 int kb_index(uint32_t kb) { return kb == 1 ? 0 : kb == 2 ? 1 : kb == 4 ? 2 : kb == 8 ? 3 : kb == 16 ? 4 : -1; }
@@ -83,13 +85,20 @@ std::mutex g_tune_mu;
 TuneEntry g_tune[5][6];
 bool g_tune_ready = false;
 
+// Measured on MI355X (whole-call time, uniform keys): u32 keys sort faster
+// with 1024 x 12 tiles below 2^22 keys, u32/u32 with 1024 x 16 tiles (two
+// workgroups per CU) below 2^27.
 TuneEntry tune_default(uint32_t kb, uint32_t vb) {
-    return TuneEntry{0u, hist_blocks_per_cu((int)kb), seg_cap((int)kb, (int)vb, kSegBins - 1)};
+    TuneEntry t{0u, hist_blocks_per_cu((int)kb), seg_cap((int)kb, (int)vb, kSegBins - 1), 0u, 0u};
+    if (kb == 4 && vb == 0) t.small_tile_cand = 1u, t.small_log2n = 22u;
+    if (kb == 4 && vb == 4) t.small_tile_cand = 1u, t.small_log2n = 27u;
+    return t;
 }
 
 bool tune_valid(uint32_t kb, uint32_t vb, const TuneEntry& t) {
     return t.tile_cand < (uint32_t)kTileCands && t.hist_bpc >= 1 && t.hist_bpc <= 8 &&
-           t.lds_threshold <= seg_cap((int)kb, (int)vb, kSegBins - 1);
+           t.lds_threshold <= seg_cap((int)kb, (int)vb, kSegBins - 1) && t.small_tile_cand < (uint32_t)kTileCands &&
+           t.small_log2n <= 40u;
 }
 
 // Reads a table file into g_tune (caller holds g_tune_mu); returns entries read or -1.
@@ -101,10 +110,11 @@ int tune_read_file(const char* path) {
     while (fgets(line, sizeof(line), f)) {
         char* h = strchr(line, '#');
         if (h) *h = 0;
-        unsigned kb, vb, tc, bpc, thr;
-        if (sscanf(line, "%u %u %u %u %u", &kb, &vb, &tc, &bpc, &thr) != 5) continue;
+        unsigned kb, vb, tc, bpc, thr, stc = 0, sl = 0;
+        const int got = sscanf(line, "%u %u %u %u %u %u %u", &kb, &vb, &tc, &bpc, &thr, &stc, &sl);
+        if (got != 5 && got != 7) continue;
         const int ki = kb_index(kb), vi = vb_index(vb);
-        const TuneEntry t{tc, bpc, thr};
+        const TuneEntry t{tc, bpc, thr, stc, sl};
         if (ki < 0 || vi < 0 || !tune_valid(kb, vb, t)) continue;
         g_tune[ki][vi] = t;
         ++n;
@@ -193,7 +203,8 @@ rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
     p->passes = p->kb;
     p->pass_kernel = pass_kernel_choice();
     const TuneEntry te = tune_get(p->kb, p->vb_core);
-    p->tile_cand = tile_cands_built((int)p->kb, (int)p->vb_core) ? te.tile_cand : 0u;
+    const uint32_t cand = (te.small_log2n && n < (1ull << te.small_log2n)) ? te.small_tile_cand : te.tile_cand;
+    p->tile_cand = tile_cands_built((int)p->kb, (int)p->vb_core) ? cand : 0u;
     p->hist_bpc = te.hist_bpc;
     if (p->pass_kernel == kPassChunk) {
         const ChunkCfg cc = chunk_cfg((int)p->kb, (int)p->vb_core);
@@ -624,6 +635,8 @@ rsort_status rsort_tuning_get(uint32_t key_bytes, uint32_t value_bytes, rsort_tu
     out->tile_candidate = t.tile_cand;
     out->hist_blocks_per_cu = t.hist_bpc;
     out->lds_threshold = t.lds_threshold;
+    out->small_tile_candidate = t.small_tile_cand;
+    out->small_log2n = t.small_log2n;
     out->reserved = 0;
     return RSORT_SUCCESS;
 }
@@ -633,10 +646,11 @@ rsort_status rsort_tuning_set(uint32_t key_bytes, uint32_t value_bytes, const rs
     const int ki = rsa::kb_index(key_bytes), vi = rsa::vb_index(value_bytes);
     if (ki < 0 || vi < 0)
         return rsa::fail(RSORT_ERROR_UNSUPPORTED, "no tuning entry for key %u / value %u bytes", key_bytes, value_bytes);
-    const rsa::TuneEntry e{t->tile_candidate, t->hist_blocks_per_cu, t->lds_threshold};
+    const rsa::TuneEntry e{t->tile_candidate, t->hist_blocks_per_cu, t->lds_threshold, t->small_tile_candidate,
+                           t->small_log2n};
     if (!rsa::tune_valid(key_bytes, value_bytes, e))
-        return rsa::fail(RSORT_ERROR_INVALID_ARGUMENT, "tuning out of range (tile candidate < %d, 1..8 histogram "
-                         "workgroups per CU, LDS threshold <= %u)", rsa::kTileCands,
+        return rsa::fail(RSORT_ERROR_INVALID_ARGUMENT, "tuning out of range (tile candidates < %d, 1..8 histogram "
+                         "workgroups per CU, LDS threshold <= %u, small_log2n <= 40)", rsa::kTileCands,
                          rsa::seg_cap((int)key_bytes, (int)value_bytes, rsa::kSegBins - 1));
     std::lock_guard<std::mutex> lk(rsa::g_tune_mu);
     rsa::tune_ensure_locked();

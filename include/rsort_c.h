@@ -185,18 +185,24 @@ rsort_status rsort_digit_counts(const rsort_desc* desc, const void* keys, uint64
  *   hist_blocks_per_cu  histogram workgroups per CU, 1..8
  *   lds_threshold       segmented sort: longest segment sorted in LDS (LDS
  *                       bins larger than it are not used), <= bin_capacity[4]
+ *   small_tile_candidate, small_log2n
+ *                       the tile candidate of sorts below 2^small_log2n keys
+ *                       (few tiles: a smaller tile fills more CUs)
  * Defaults are built in; at first use the library reads
  * $RSORT_AMD_TUNING_FILE, else tuning_gfx950.txt next to the library (lines
- * "key_bytes value_bytes tile_candidate hist_blocks_per_cu lds_threshold").
+ * "key_bytes value_bytes tile_candidate hist_blocks_per_cu lds_threshold
+ * [small_tile_candidate small_log2n]").
  * Changing an entry changes workspace sizes: query again after a change.
  * The reference fixes these per build in a table for sm_86
  * (include/rsort/detail/tuning.cuh:138-229); there is no reference tuner.
  */
 typedef struct rsort_tuning {
-    uint32_t tile_candidate;
+    uint32_t tile_candidate;       /* sorts of n >= 2^small_log2n keys */
     uint32_t hist_blocks_per_cu;
     uint32_t lds_threshold;
-    uint32_t reserved; /* must be 0 */
+    uint32_t small_tile_candidate; /* sorts of n < 2^small_log2n keys */
+    uint32_t small_log2n;          /* 0: tile_candidate for every n */
+    uint32_t reserved;             /* must be 0 */
 } rsort_tuning;
 
 rsort_status rsort_tuning_get(uint32_t key_bytes, uint32_t value_bytes, rsort_tuning* out);

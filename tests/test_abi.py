@@ -184,8 +184,12 @@ def test_tuning_defaults_set_and_plan(rs, tmp_path):
     rs.tuning_load("")  # built-in defaults only
     t = rs.tuning_get(4, 4)
     assert (t.tile_candidate, t.lds_threshold) == (0, 16384) and 1 <= t.hist_blocks_per_cu <= 8
+    assert (t.small_tile_candidate, t.small_log2n) == (1, 27)
     d = rs.make_desc(0, 4, 4)
-    base = rs.plan(d, 1 << 20)
+    # below 2^27 keys the small-sort candidate (1024 x 16) is used
+    small = rs.plan(d, (1 << 27) - 1)
+    assert (small.threads, small.items_per_thread) == (1024, 16)
+    base = rs.plan(d, 1 << 27)
     assert (base.threads, base.items_per_thread) == (1024, 20)
     try:
         rs.tuning_set(4, 4, 1, 2, 8192)
@@ -204,15 +208,18 @@ def test_tuning_defaults_set_and_plan(rs, tmp_path):
             1024, 16)
         # a table file round trip
         f = tmp_path / "t.txt"
-        f.write_text("# kb vb tile hist lds\n4 4 1 3 4096\n8 0 2 2 8192\nbad line\n4 4 9 1 1\n")
+        f.write_text("# kb vb tile hist lds [small_tile small_log2n]\n4 4 1 3 4096\n8 0 2 2 8192 1 24\nbad line\n"
+                     "4 4 9 1 1\n8 8 0 1 1 0\n")
         assert rs.tuning_load(str(f)) == 2
         t = rs.tuning_get(4, 4)
-        assert (t.tile_candidate, t.hist_blocks_per_cu, t.lds_threshold) == (1, 3, 4096)
-        assert rs.tuning_get(8, 0).tile_candidate == 2
+        assert (t.tile_candidate, t.hist_blocks_per_cu, t.lds_threshold, t.small_log2n) == (1, 3, 4096, 0)
+        t8 = rs.tuning_get(8, 0)
+        assert (t8.tile_candidate, t8.small_tile_candidate, t8.small_log2n) == (2, 1, 24)
+        assert rs.plan(rs.make_desc(0, 8, 0), 1 << 23).threads == rs.plan(rs.make_desc(0, 8, 0), 1 << 25).threads * 2
         assert rs.tuning_load(str(tmp_path / "missing.txt")) == -1
     finally:
         rs.tuning_load(None)
-    assert rs.plan(d, 1 << 20).items_per_thread == 20
+    assert rs.plan(d, 1 << 20).items_per_thread == 16 and rs.plan(d, 1 << 28).items_per_thread == 20
 
 
 def test_persisted_table_is_read_at_first_use(rs):
@@ -222,13 +229,16 @@ def test_persisted_table_is_read_at_first_use(rs):
     path = os.path.join(os.path.dirname(rs.LIB_PATH), "tuning_gfx950.txt")
     rows = [l.split() for l in open(path) if l.strip() and not l.startswith("#")]
     assert rs.tuning_load(None) == len(rows) > 0
-    for kb, vb, tc, bpc, thr in rows:
-        t = rs.tuning_get(int(kb), int(vb))
-        assert (t.tile_candidate, t.hist_blocks_per_cu, t.lds_threshold) == (int(tc), int(bpc), int(thr))
+    for row in rows:
+        kb, vb, tc, bpc, thr = (int(x) for x in row[:5])
+        stc, sl = (int(row[5]), int(row[6])) if len(row) == 7 else (0, 0)
+        t = rs.tuning_get(kb, vb)
+        assert (t.tile_candidate, t.hist_blocks_per_cu, t.lds_threshold) == (tc, bpc, thr)
+        assert (t.small_tile_candidate, t.small_log2n) == (stc, sl)
 
 
 @pytest.mark.parametrize("args", [(4, 4, 3, 1, 0), (4, 4, 0, 0, 0), (4, 4, 0, 9, 0), (4, 4, 0, 1, 16385),
-                                  (3, 4, 0, 1, 0), (4, 12, 0, 1, 0)])
+                                  (3, 4, 0, 1, 0), (4, 12, 0, 1, 0), (4, 4, 0, 1, 0, 3, 20), (4, 4, 0, 1, 0, 1, 41)])
 def test_tuning_rejects_out_of_range(rs, args):
     with pytest.raises(rs.RsortError):
         rs.tuning_set(*args)
