@@ -534,6 +534,31 @@ hipError_t launch_gather(void* dst, const void* src, const uint32_t* idx, uint64
     return hipGetLastError();
 }
 
+// Keys of n records (record_bytes apart, key at key_offset) into dst, in the
+// widest unit that the record base, stride, offset and key width allow.
+hipError_t launch_extract(void* dst, const void* rec, uint64_t n, uint32_t record_bytes, uint32_t key_offset,
+                          uint32_t key_bytes, int num_cus, hipStream_t s) {
+    const uintptr_t al = (uintptr_t)rec | record_bytes | key_offset | key_bytes;
+    const dim3 grid(grid_for(n * key_bytes, 4096, num_cus, 8)), block(256);
+    if ((al & 15u) == 0) {
+        hipLaunchKernelGGL(k_extract_keys<uint4>, grid, block, 0, s, (uint4*)dst, (const uint4*)rec, n,
+                           record_bytes / 16u, key_offset / 16u, key_bytes / 16u);
+    } else if ((al & 7u) == 0) {
+        hipLaunchKernelGGL(k_extract_keys<uint64_t>, grid, block, 0, s, (uint64_t*)dst, (const uint64_t*)rec, n,
+                           record_bytes / 8u, key_offset / 8u, key_bytes / 8u);
+    } else if ((al & 3u) == 0) {
+        hipLaunchKernelGGL(k_extract_keys<uint32_t>, grid, block, 0, s, (uint32_t*)dst, (const uint32_t*)rec, n,
+                           record_bytes / 4u, key_offset / 4u, key_bytes / 4u);
+    } else if ((al & 1u) == 0) {
+        hipLaunchKernelGGL(k_extract_keys<uint16_t>, grid, block, 0, s, (uint16_t*)dst, (const uint16_t*)rec, n,
+                           record_bytes / 2u, key_offset / 2u, key_bytes / 2u);
+    } else {
+        hipLaunchKernelGGL(k_extract_keys<uint8_t>, grid, block, 0, s, (uint8_t*)dst, (const uint8_t*)rec, n,
+                           record_bytes, key_offset, key_bytes);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, int num_cus, hipStream_t s) {
     const uintptr_t al = (uintptr_t)dst | (uintptr_t)src | (uintptr_t)bytes;
     const dim3 block(256);
@@ -557,6 +582,46 @@ extern "C" {
 rsort_status rsort_sort(const rsort_desc* desc, void* keys, void* values, void* workspace, size_t* temp_bytes,
                         uint64_t n, void* stream) {
     return rsa::run_sort(desc, keys, values, workspace, temp_bytes, n, stream, nullptr, 0, nullptr);
+}
+
+// Records sorted by one key field: the keys are copied out of the records
+// into the workspace, then sorted as keys with the whole record as the value
+// (records of a power-of-two size up to 16 bytes move directly, others by a
+// 32-bit index and a gather), so the record's own key bytes move with it.
+rsort_status rsort_sort_records(const rsort_desc* desc, void* records, uint64_t n, uint32_t record_bytes,
+                                uint32_t key_offset, void* workspace, size_t* temp_bytes, void* stream) {
+    if (!desc) return rsa::fail(RSORT_ERROR_INVALID_ARGUMENT, "descriptor is NULL");
+    if (desc->value_bytes)
+        return rsa::fail(RSORT_ERROR_INVALID_ARGUMENT, "value_bytes must be 0: the record is the value");
+    if (!temp_bytes) return rsa::fail(RSORT_ERROR_INVALID_ARGUMENT, "temp_bytes is NULL");
+    if (record_bytes == 0 || (uint64_t)key_offset + desc->key_bytes > record_bytes)
+        return rsa::fail(RSORT_ERROR_INVALID_ARGUMENT, "key of %u bytes at offset %u does not fit a %u-byte record",
+                         desc->key_bytes, key_offset, record_bytes);
+    rsort_desc d = *desc;
+    d.value_bytes = record_bytes;
+    size_t inner = 0;
+    rsort_status st = rsa::run_sort(&d, nullptr, nullptr, nullptr, &inner, n, stream, nullptr, 0, nullptr);
+    if (st != RSORT_SUCCESS) return st;
+    const size_t keys_bytes = ((size_t)n * desc->key_bytes + 255u) & ~(size_t)255u;
+    const size_t total = 256u + keys_bytes + inner;
+    if (!workspace) {
+        *temp_bytes = total;
+        return RSORT_SUCCESS;
+    }
+    if (*temp_bytes < total)
+        return rsa::fail(RSORT_ERROR_WORKSPACE_TOO_SMALL, "workspace of %zu bytes, %zu required", *temp_bytes, total);
+    if (n <= 1) return RSORT_SUCCESS;
+    if (!records) return rsa::fail(RSORT_ERROR_INVALID_ARGUMENT, "records is NULL");
+    if ((uintptr_t)workspace % 16) return rsa::fail(RSORT_ERROR_INVALID_ARGUMENT, "workspace not 16-byte aligned");
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return rsa::hip_fail(e, "hipGetDevice");
+    uint8_t* keys = (uint8_t*)(((uintptr_t)workspace + 255u) & ~(uintptr_t)255u);
+    e = rsa::launch_extract(keys, records, n, record_bytes, key_offset, desc->key_bytes, rsa::device_cus(dev),
+                            (hipStream_t)stream);
+    if (e != hipSuccess) return rsa::hip_fail(e, "key extraction");
+    size_t inner_bytes = total - (size_t)(keys + keys_bytes - (uint8_t*)workspace);
+    return rsa::run_sort(&d, keys, records, keys + keys_bytes, &inner_bytes, n, stream, nullptr, 0, nullptr);
 }
 
 rsort_status rsort_sort_profiled(const rsort_desc* desc, void* keys, void* values, void* workspace,

@@ -368,14 +368,37 @@ static __global__ void k_fill_zero16(uint4* __restrict__ p, uint64_t units) {
 // Odd number of executed passes leaves the result in the workspace buffer:
 // copy it back (radix_kernel.cuh:468-473), decided on the device because
 // uniform passes are skipped on the device.
+// Copies `elems` elements of E; in 16-byte units when both pointers are
+// 16-byte aligned (`vec`), element by element otherwise.
 template <typename E>
-__global__ void k_copy_back(const Ctrl* __restrict__ ctrl, uint32_t passes, E* __restrict__ dst,
-                            const E* __restrict__ src, uint64_t elems) {
+__device__ __forceinline__ void copy_elems(E* __restrict__ dst, const E* __restrict__ src, uint64_t elems, bool vec,
+                                           uint64_t t0, uint64_t stride) {
+    uint64_t done = 0;
+    if (vec) {
+        constexpr uint64_t PER = 16u / sizeof(E);
+        const uint64_t units = elems / PER;
+        const uint4* s4 = (const uint4*)src;
+        uint4* d4 = (uint4*)dst;
+        for (uint64_t i = t0; i < units; i += stride) d4[i] = s4[i];
+        done = units * PER;
+    }
+    for (uint64_t i = done + t0; i < elems; i += stride) dst[i] = src[i];
+}
+
+// Keys (and values) back into the caller's buffers, in one launch; the grid
+// is bounded (the common case, an even executed-pass count, is a no-op).
+template <typename KE, typename VE>
+__global__ __launch_bounds__(256) void k_copy_back(const Ctrl* __restrict__ ctrl, uint32_t passes,
+                                                   KE* __restrict__ dk, const KE* __restrict__ sk,
+                                                   VE* __restrict__ dv, const VE* __restrict__ sv, uint64_t elems,
+                                                   uint32_t vec_k, uint32_t vec_v) {
     const uint32_t pmask = passes >= 32 ? ~0u : ((1u << passes) - 1u);
     const uint32_t active = ~ctrl->skip_mask & pmask;
     if ((__builtin_popcount(active) & 1u) == 0u) return;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < elems; i += stride) dst[i] = src[i];
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    copy_elems(dk, sk, elems, vec_k != 0u, t0, stride);
+    if (dv) copy_elems(dv, sv, elems, vec_v != 0u, t0, stride);
 }
 
 template <typename E>
@@ -387,6 +410,20 @@ __global__ void k_copy(E* __restrict__ dst, const E* __restrict__ src, uint64_t 
 static __global__ void k_iota(uint32_t* __restrict__ p, uint64_t n) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = (uint32_t)i;
+}
+
+// Keys of records: dst key i (kel elements of E) = the kel elements at
+// element offset `off` of record i (rel elements of E).
+template <typename E>
+__global__ void k_extract_keys(E* __restrict__ dst, const E* __restrict__ rec, uint64_t n, uint32_t rel,
+                               uint32_t off, uint32_t kel) {
+    const uint64_t total = n * kel;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+        const uint64_t i = t / kel;
+        const uint32_t c = (uint32_t)(t - i * kel);
+        dst[t] = rec[i * rel + off + c];
+    }
 }
 
 // dst record i = src record idx[i]; records of `rec` elements of type E.

@@ -109,6 +109,8 @@ hipError_t launch_iota(uint32_t* p, uint64_t n, int num_cus, hipStream_t s);
 hipError_t launch_gather(void* dst, const void* src, const uint32_t* idx, uint64_t n, uint32_t rec_bytes,
                          int num_cus, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, int num_cus, hipStream_t s);
+hipError_t launch_extract(void* dst, const void* rec, uint64_t n, uint32_t record_bytes, uint32_t key_offset,
+                          uint32_t key_bytes, int num_cus, hipStream_t s);
 
 inline uint32_t grid_for(uint64_t work, uint32_t per_block, int num_cus, uint32_t blocks_per_cu) {
     uint64_t g = (work + per_block - 1) / per_block;

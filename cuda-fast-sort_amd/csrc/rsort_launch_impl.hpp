@@ -86,13 +86,14 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
 
     // 4. copy back when the executed pass count is odd (decided on device)
     if ((P & 1u) || j.allow_skip) {
-        const uint32_t g = grid_for(j.n, 256u * 4u, j.num_cus, 8);
-        hipLaunchKernelGGL((k_copy_back<KM>), dim3(g), dim3(256), 0, s, (const Ctrl*)j.ctrl, P, (KM*)j.keys,
-                           (const KM*)j.tmp_keys, j.n);
-        if constexpr (VB > 0) {
-            hipLaunchKernelGGL((k_copy_back<VM>), dim3(g), dim3(256), 0, s, (const Ctrl*)j.ctrl, P, (VM*)j.vals,
-                               (const VM*)j.tmp_vals, j.n);
-        }
+        using VE = std::conditional_t<(VB > 0), VM, uint8_t>;
+        const uint64_t bytes = j.n * (uint64_t)(KB > VB ? KB : VB);
+        const uint32_t g = grid_for(bytes / 16u + 1u, 256u * 8u, j.num_cus, 4);
+        const uint32_t vk = (((uintptr_t)j.keys | (uintptr_t)j.tmp_keys) & 15u) == 0u;
+        const uint32_t vv = VB > 0 && (((uintptr_t)j.vals | (uintptr_t)j.tmp_vals) & 15u) == 0u;
+        hipLaunchKernelGGL((k_copy_back<KM, VE>), dim3(g), dim3(256), 0, s, (const Ctrl*)j.ctrl, P, (KM*)j.keys,
+                           (const KM*)j.tmp_keys, VB > 0 ? (VE*)j.vals : (VE*)nullptr,
+                           VB > 0 ? (const VE*)j.tmp_vals : (const VE*)nullptr, j.n, vk, vv);
         record_event(j, s);
     }
     return hipGetLastError();

@@ -130,6 +130,9 @@ def lib() -> ctypes.CDLL:
         L.rsort_plan.argtypes = [ctypes.POINTER(Desc), u64, ctypes.POINTER(PlanInfo)]
         L.rsort_plan.restype = i32
         u32 = ctypes.c_uint32
+        L.rsort_sort_records.argtypes = [ctypes.POINTER(Desc), vp, u64, u32, u32, vp, ctypes.POINTER(ctypes.c_size_t),
+                                         vp]
+        L.rsort_sort_records.restype = i32
         L.rsort_segmented_sort.argtypes = [ctypes.POINTER(Desc), vp, vp, u64, vp, u32, u64, vp,
                                            ctypes.POINTER(ctypes.c_size_t), vp]
         L.rsort_segmented_sort.restype = i32
@@ -223,6 +226,30 @@ def sort_raw(desc: Desc, keys_ptr: int | None, vals_ptr: int | None, ws_ptr: int
     tb = ctypes.c_size_t(temp_bytes or 0)
     _check(lib().rsort_sort(ctypes.byref(desc), keys_ptr, vals_ptr, ws_ptr, ctypes.byref(tb), n, _stream_ptr(stream)))
     return int(tb.value)
+
+
+def sort_records_raw(desc: Desc, records_ptr: int | None, n: int, record_bytes: int, key_offset: int,
+                     ws_ptr: int | None, temp_bytes: int | None, stream=None) -> int:
+    """Direct call of rsort_sort_records; returns the workspace size (query) or temp_bytes."""
+    tb = ctypes.c_size_t(temp_bytes or 0)
+    _check(lib().rsort_sort_records(ctypes.byref(desc), records_ptr, n, record_bytes, key_offset, ws_ptr,
+                                    ctypes.byref(tb), _stream_ptr(stream)))
+    return int(tb.value)
+
+
+def sort_records(records, record_bytes: int, key_offset: int, key_type: "KeyType", temp_bytes, d_workspace,
+                 descending: bool = False, stream=None):
+    """Array-of-structs sort: `records` is a device tensor holding n records of
+    record_bytes bytes; sorts them stably in place by the key_type field at
+    key_offset.  d_workspace None -> returns the workspace size."""
+    nbytes = records.numel() * records.element_size()
+    if nbytes % record_bytes:
+        raise ValueError("records tensor is not a whole number of records")
+    n = nbytes // record_bytes
+    d = make_desc(key_type.kind, key_type.nbytes, 0, descending)
+    if d_workspace is None:
+        return sort_records_raw(d, None, n, record_bytes, key_offset, None, 0)
+    return sort_records_raw(d, _ptr(records), n, record_bytes, key_offset, _ptr(d_workspace), temp_bytes, stream)
 
 
 def plan(desc: Desc, n: int) -> PlanInfo:

@@ -71,6 +71,29 @@ inline void call(Key_T* keys, Value_T* values, size_t* temp_bytes, uint8_t* work
                                    static_cast<void*>(stream));
 }
 
+// Byte offset of a data member, from its address in a record-sized buffer
+// (no Record object is constructed).
+template <typename Record, typename Key_T>
+inline uint32_t member_offset(Key_T Record::*key) {
+    alignas(Record) static unsigned char buf[sizeof(Record)];
+    const Record* r = reinterpret_cast<const Record*>(buf);
+    return (uint32_t)(reinterpret_cast<const unsigned char*>(&(r->*key)) - buf);
+}
+
+template <bool Descending, typename Record, typename Key_T>
+inline void call_records(Record* records, Key_T Record::*key, size_t* temp_bytes, uint8_t* workspace, size_t n,
+                         hipStream_t stream) {
+    static_assert(std::is_trivially_copyable<Record>::value, "records are moved as bytes");
+    rsort_desc d{};
+    d.key_kind = key_desc<Key_T>::kind;
+    d.key_bytes = key_desc<Key_T>::bytes;
+    d.value_bytes = 0;
+    d.descending = Descending ? 1u : 0u;
+    last_status_ref() = rsort_sort_records(&d, static_cast<void*>(records), (uint64_t)n, (uint32_t)sizeof(Record),
+                                           member_offset(key), static_cast<void*>(workspace), temp_bytes,
+                                           static_cast<void*>(stream));
+}
+
 }  // namespace detail
 
 // Outcome of the calling thread's last rsort call.
@@ -102,6 +125,23 @@ inline void sort_pairs_descending(Key_T* d_inout, Value_T* d_inout_values, size_
                                   Len_T n, hipStream_t stream = nullptr) {
     static_assert(std::is_integral_v<Len_T>, "Type of N (Len_T) must be an integral type");
     detail::call<true, Key_T, Value_T>(d_inout, d_inout_values, temp_bytes, d_workspace, (size_t)n, stream);
+}
+
+// Array-of-structs sort (no counterpart in the reference's API; it plans
+// one, rsort.cuh:95-102): sorts d_records stably by the member `key`, e.g.
+//   struct Particle { float x, y; uint32_t cell; };
+//   rsort::sort_records(d_particles, &Particle::cell, &temp_bytes, d_ws, n);
+// Record must be trivially copyable; the member's type must be a key type.
+template <typename Record, typename Key_T, typename Len_T>
+inline void sort_records(Record* d_records, Key_T Record::*key, size_t* temp_bytes, uint8_t* d_workspace, Len_T n,
+                         hipStream_t stream = nullptr) {
+    detail::call_records<false>(d_records, key, temp_bytes, d_workspace, (size_t)n, stream);
+}
+
+template <typename Record, typename Key_T, typename Len_T>
+inline void sort_records_descending(Record* d_records, Key_T Record::*key, size_t* temp_bytes, uint8_t* d_workspace,
+                                    Len_T n, hipStream_t stream = nullptr) {
+    detail::call_records<true>(d_records, key, temp_bytes, d_workspace, (size_t)n, stream);
 }
 
 }  // namespace rsort

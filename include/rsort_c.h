@@ -88,6 +88,24 @@ rsort_status rsort_sort_profiled(const rsort_desc* desc, void* keys, void* value
                                  size_t* temp_bytes, uint64_t n, void* stream, float* stage_ms, int max_stages,
                                  int* n_stages);
 
+/*
+ * Array-of-structs sort: n records of record_bytes each, sorted stably in
+ * place by the key field of desc->key_bytes bytes at key_offset in every
+ * record (desc->value_bytes must be 0: the rest of the record is the value).
+ * Same query / workspace / stream convention as rsort_sort.  The keys are
+ * copied out of the records into the workspace, then the records move as the
+ * values of that key array (directly for record sizes 1/2/4/8/16, by a 32-bit
+ * index and a gather otherwise, which needs n < 2^32).  Records of a
+ * power-of-two size up to 16 must be aligned to that size.
+ *
+ * Replaces the reference's planned AoS interface: rsort.cuh:95-102 (AoS input
+ * converted to SoA) and its C++26 reflection prototype, which splits a record
+ * into its key member and the remaining members
+ * (include/rsort/experimental/reflection.h:150-184).
+ */
+rsort_status rsort_sort_records(const rsort_desc* desc, void* records, uint64_t n, uint32_t record_bytes,
+                                uint32_t key_offset, void* workspace, size_t* temp_bytes, void* stream);
+
 /* Plan of a sort: what the host would launch for (desc, n).  Host only. */
 typedef struct rsort_plan_info {
     uint32_t passes;          /* = key_bytes */

@@ -116,6 +116,54 @@ static void run_case(const char* name, size_t n) {
     CHECK_HIP(hipFree(dk));
 }
 
+// Array-of-structs sort (rsort::sort_records): a 24-byte record sorted by a
+// float member (ascending) or by a uint16_t member (descending); each record
+// must equal the std::stable_sort of the host records by that member.
+struct Particle {
+    double x;
+    float y;
+    uint16_t tag;
+    uint32_t id;
+};
+
+template <bool ByTag>
+static void run_records(const char* name, size_t n) {
+    std::mt19937_64 g(n * 17 + ByTag);
+    std::vector<Particle> h(n);
+    for (size_t i = 0; i < n; ++i) {
+        h[i].x = (double)(g() % 1000);
+        h[i].y = (float)((int)(g() % 2001) - 1000) * 0.5f;
+        h[i].tag = (uint16_t)(g() % 300);
+        h[i].id = (uint32_t)i;
+    }
+    Particle* d = nullptr;
+    uint8_t* ws = nullptr;
+    size_t temp_bytes = 0;
+    CHECK_HIP(hipMalloc(&d, n * sizeof(Particle)));
+    CHECK_HIP(hipMemcpy(d, h.data(), n * sizeof(Particle), hipMemcpyHostToDevice));
+    for (int phase = 0; phase < 2; ++phase) {
+        if constexpr (ByTag) rsort::sort_records_descending(d, &Particle::tag, &temp_bytes, ws, n);
+        else rsort::sort_records(d, &Particle::y, &temp_bytes, ws, n);
+        if (rsort::last_status() != RSORT_SUCCESS) {
+            printf("FAIL %s: %s\n", name, rsort::last_error());
+            ++failures;
+            return;
+        }
+        if (phase == 0) CHECK_HIP(hipMalloc(&ws, temp_bytes));
+    }
+    CHECK_HIP(hipDeviceSynchronize());
+    std::vector<Particle> r(n);
+    CHECK_HIP(hipMemcpy(r.data(), d, n * sizeof(Particle), hipMemcpyDeviceToHost));
+    std::stable_sort(h.begin(), h.end(), [](const Particle& a, const Particle& b) {
+        return ByTag ? a.tag > b.tag : a.y < b.y;
+    });
+    const bool ok = memcmp(r.data(), h.data(), n * sizeof(Particle)) == 0;
+    printf("%s %s n=%zu\n", ok ? "ok  " : "FAIL", name, n);
+    failures += ok ? 0 : 1;
+    CHECK_HIP(hipFree(ws));
+    CHECK_HIP(hipFree(d));
+}
+
 int main() {
     for (size_t n : {(size_t)1000, (size_t)123457}) {
         run_case<uint32_t, uint32_t, false, false>("sort<uint32_t>", n);
@@ -129,6 +177,10 @@ int main() {
         run_case<unsigned __int128, uint64_t, false, true>("sort_pairs<unsigned __int128,uint64_t>", n);
         run_case<__int128, uint32_t, true, true>("sort_pairs_descending<__int128,uint32_t>", n);
         run_case<long double, uint32_t, false, true>("sort_pairs<long double,uint32_t>", n);
+    }
+    for (size_t n : {(size_t)1000, (size_t)200003}) {
+        run_records<false>("sort_records<Particle, &Particle::y>", n);
+        run_records<true>("sort_records_descending<Particle, &Particle::tag>", n);
     }
     printf("%d failures\n", failures);
     return failures ? 1 : 0;

@@ -242,3 +242,26 @@ def test_persisted_table_is_read_at_first_use(rs):
 def test_tuning_rejects_out_of_range(rs, args):
     with pytest.raises(rs.RsortError):
         rs.tuning_set(*args)
+
+
+# ---------------------------------------------------------------------------
+# Array-of-structs sort: size query and argument checks (no device work)
+# ---------------------------------------------------------------------------
+def test_sort_records_query_and_arguments(rs):
+    import ctypes
+
+    d = rs.make_desc(0, 4, 0)
+    tb = rs.sort_records_raw(d, None, 1 << 20, 12, 4, None, 0)
+    # extracted keys + a key/value sort of 12-byte values (index mode)
+    inner = rs.sort_raw(rs.make_desc(0, 4, 12), None, None, None, 0, 1 << 20)
+    assert tb >= inner + (1 << 20) * 4
+    for args, why in (((rs.make_desc(0, 4, 4), 12, 4), "value_bytes"), ((d, 12, 9), "key past the record"),
+                      ((d, 0, 0), "empty record"), ((rs.make_desc(0, 8, 0), 6, 0), "key wider than the record")):
+        desc, rb, off = args
+        with pytest.raises(rs.RsortError):
+            rs.sort_records_raw(desc, None, 100, rb, off, None, 0)
+    # too-small workspace is rejected before any device work
+    t = ctypes.c_size_t(16)
+    st = rs.lib().rsort_sort_records(ctypes.byref(d), ctypes.c_void_p(0x1000), 1000, 12, 0, ctypes.c_void_p(0x1000),
+                                     ctypes.byref(t), None)
+    assert st == 3
