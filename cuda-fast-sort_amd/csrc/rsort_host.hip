@@ -170,7 +170,7 @@ struct Plan {
     uint64_t tile, num_tiles;
     size_t off_tmp_keys, off_tmp_vals, off_idx, off_vals_copy, off_ctrl, off_counts, off_gp, off_lb;
     size_t off_hist_partials;
-    size_t lb_bytes, total;
+    size_t lb_bytes, lb_group_off, total;
 };
 
 rsort_status check_desc(const rsort_desc* d) {
@@ -228,7 +228,10 @@ rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
     // fill kernel clears them at the start of every sort
     p->off_ctrl = reserve(&off, sizeof(Ctrl));
     p->off_counts = reserve(&off, (size_t)p->passes * kRadix * off_sz, 16);
-    p->lb_bytes = (size_t)p->num_tiles * kRadix * off_sz;
+    // lookback: tile words, then per-pass group words (kLbGroup tiles each)
+    const size_t groups = (size_t)((p->num_tiles + kLbGroup - 1) / kLbGroup);
+    p->lb_group_off = (size_t)p->num_tiles * kRadix * off_sz;
+    p->lb_bytes = p->lb_group_off + (size_t)p->passes * groups * kRadix * off_sz;
     p->off_lb = reserve(&off, p->lb_bytes, 16);
     if (p->off_counts != p->off_ctrl + sizeof(Ctrl) || p->off_lb != p->off_counts + (size_t)p->passes * kRadix * off_sz)
         return fail(RSORT_ERROR_UNSUPPORTED, "workspace layout: cleared region is not contiguous");
@@ -399,6 +402,7 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
     j.hist_partials = (uint32_t*)(base + p.off_hist_partials);
     j.lookback = base + p.off_lb;
     j.lb_bytes = p.lb_bytes;
+    j.lb_groups = base + p.off_lb + p.lb_group_off;
     j.n = n;
     j.num_tiles = p.num_tiles;
     j.passes = p.passes;

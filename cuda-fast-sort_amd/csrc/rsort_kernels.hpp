@@ -342,6 +342,7 @@ struct PassParams {
     void* vals_b;
     const void* gp;        // [passes][256] exclusive digit offsets
     void* lookback;        // [tiles][256] words
+    void* lb_groups;       // [passes][groups][256] group words (kLbGroup tiles each), zeroed per sort
     Ctrl* ctrl;
     uint64_t n;
     uint32_t num_tiles;
@@ -858,6 +859,8 @@ __global__ __launch_bounds__(THREADS, pass_min_waves(KB, VB, K)) void k_onesweep
     g.kend = a.n;
     const Off* gp_row = (const Off*)a.gp + a.pass * 256u;
     Off* lb = (Off*)a.lookback;
+    constexpr bool LB_GROUPS = true;
+    Off* lbg = (Off*)a.lb_groups + (size_t)a.pass * ((a.num_tiles + kLbGroup - 1) / kLbGroup) * 256u;
     const TwMasks& tw = a.tw;
     const uint32_t tile_tid = threadIdx.x;
 #include "rsort_onesweep_tile.inc"

@@ -31,7 +31,8 @@ struct SortJob {
     void* gp;
     uint32_t* hist_partials;    // [hist grid][passes][256]
     void* lookback;
-    uint64_t lb_bytes;
+    uint64_t lb_bytes;    // tile words and group words (cleared together)
+    void* lb_groups;      // group words: [passes][groups][256]
     uint64_t n;
     uint64_t num_tiles;
     uint32_t passes;

@@ -55,6 +55,7 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     pp.vals_b = j.tmp_vals;
     pp.gp = j.gp;
     pp.lookback = j.lookback;
+    pp.lb_groups = j.lb_groups;
     pp.ctrl = j.ctrl;
     pp.n = j.n;
     pp.num_tiles = (uint32_t)j.num_tiles;

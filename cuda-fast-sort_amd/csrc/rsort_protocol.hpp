@@ -172,6 +172,128 @@ RSA_HD W lookback_walk(int64_t tile, uint32_t epoch, Load&& load, Backoff&& back
 }
 
 // ---------------------------------------------------------------------------
+// Group aggregates (plain sorts): tiles are also summed per group of
+// kLbGroup consecutive tiles.  Each tile adds (1 << count shift) | aggregate
+// to its group's word for every digit with a device-scope atomic when it
+// publishes its aggregate, so a group word with count == kLbGroup holds the
+// group's total.  One group word per (pass, group, digit), zeroed per sort.
+// A walk that reaches a complete group adds its total in one read instead of
+// reading its kLbGroup tile words, and a group whose last tile has published
+// an inclusive prefix ends the walk.
+// ---------------------------------------------------------------------------
+constexpr int kLbGroup = 8;
+
+template <typename W>
+struct GrpLayout {
+    static constexpr int kCountShift = (int)sizeof(W) * 8 - 4;
+    static constexpr W kSumMask = (W(1) << kCountShift) - W(1);
+};
+
+template <typename W>
+RSA_HD W grp_add(W aggregate) {
+    return (W(1) << GrpLayout<W>::kCountShift) | aggregate;
+}
+
+template <typename W>
+RSA_HD bool grp_complete(W w) {
+    return (uint32_t)(w >> GrpLayout<W>::kCountShift) == (uint32_t)kLbGroup;
+}
+
+template <typename W>
+RSA_HD W grp_sum(W w) {
+    return w & GrpLayout<W>::kSumMask;
+}
+
+// Adds the tile words of tiles hi, hi-1, ..., lo (ready ones; a not-ready
+// word is polled again after backoff()) until an inclusive prefix is met.
+// Returns true when one was met (acc then holds the finished prefix).
+template <typename W, int WIN, typename Load, typename Backoff>
+RSA_HD bool lookback_range(int64_t hi, int64_t lo, uint32_t epoch, Load&& load, Backoff&& backoff, W& acc,
+                           uint32_t& steps, uint32_t& spins) {
+    int64_t j = hi;  // next tile to consume
+    while (j >= lo) {
+        W word[WIN];
+#pragma unroll
+        for (int k = 0; k < WIN; ++k) word[k] = (j - k) >= lo ? load(j - k) : (W)0;
+        int used = 0;
+        bool stop = false;
+#pragma unroll
+        for (int k = 0; k < WIN; ++k) {
+            if (!stop && (j - k) >= lo) {
+                if (!lb_ready(word[k], epoch)) {
+                    stop = true;
+                } else {
+                    acc += lb_value(word[k]);
+                    used = k + 1;
+                    if (lb_flag(word[k]) == kLbPrefix) {
+                        steps += (uint32_t)used;
+                        return true;
+                    }
+                }
+            }
+        }
+        steps += (uint32_t)used;
+        j -= used;
+        if (stop) {
+            ++spins;
+            backoff();
+        }
+    }
+    return false;
+}
+
+// Exclusive prefix of `tile` over the earlier tiles with group aggregates:
+// the earlier tiles of its own group at tile level, then whole groups, GW per
+// round (the last tile's word and the group word of each), falling back to
+// the tile words of a group that is not complete yet.  Tile 0 publishes an
+// inclusive prefix and group 0 is complete once its tiles have published, so
+// every walk ends.
+template <typename W, int WIN, int GW, typename LoadTile, typename LoadGroup, typename Backoff>
+RSA_HD W lookback_walk_grouped(int64_t tile, uint32_t epoch, LoadTile&& load_tile, LoadGroup&& load_group,
+                               Backoff&& backoff, uint32_t& steps, uint32_t& spins) {
+    W acc = 0;
+    const int64_t g = tile / kLbGroup;
+    if (lookback_range<W, WIN>(tile - 1, g * kLbGroup, epoch, load_tile, backoff, acc, steps, spins)) return acc;
+    int64_t h = g - 1;  // next group to consume
+    while (h >= 0) {
+        W last[GW], grp[GW];
+#pragma unroll
+        for (int k = 0; k < GW; ++k) {
+            const bool in = (h - k) >= 0;
+            last[k] = in ? load_tile((h - k) * kLbGroup + (kLbGroup - 1)) : (W)0;
+            grp[k] = in ? load_group(h - k) : (W)0;
+        }
+        int used = 0;
+        bool fallback = false;
+#pragma unroll
+        for (int k = 0; k < GW; ++k) {
+            if (!fallback && (h - k) >= 0) {
+                if (lb_ready(last[k], epoch) && lb_flag(last[k]) == kLbPrefix) {
+                    steps += (uint32_t)(2 * (k + 1));
+                    return acc + lb_value(last[k]);
+                }
+                if (grp_complete(grp[k])) {
+                    acc += grp_sum(grp[k]);
+                    used = k + 1;
+                } else {
+                    fallback = true;
+                }
+            }
+        }
+        steps += (uint32_t)(2 * used);
+        h -= used;
+        if (fallback) {
+            // group h is not complete yet: its tile words, polling as needed
+            if (lookback_range<W, WIN>(h * kLbGroup + (kLbGroup - 1), h * kLbGroup, epoch, load_tile, backoff, acc,
+                                       steps, spins))
+                return acc;
+            --h;
+        }
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------------------
 // Key twiddle masks (runtime, wave-uniform).  For a key of W bits held in an
 // unsigned word u:
 //   in : t = u ^ ((sext(msb(u)) & fmask) | smask) ^ dmask

@@ -469,6 +469,8 @@ __global__ __launch_bounds__(THREADS) void k_seg_onesweep(SegParams a) {
     constexpr uint32_t TILE = (uint32_t)THREADS * K;
     constexpr int LB_W0 = RSA_LB_W0;
     constexpr int LB_WMAX = RSA_LB_WMAX;
+    constexpr bool LB_GROUPS = false;  // each segment heads its own chain
+    Off* lbg = nullptr;
     __shared__ OnesweepLds<KB, VB, WIDE, THREADS, K> L;
 
     const uint32_t pass = a.pass;
