@@ -729,6 +729,10 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
 #ifndef RSA_LB_WMAX
 #define RSA_LB_WMAX 8
 #endif
+// Groups read per round by the grouped walk (two words each).
+#ifndef RSA_LB_GW
+#define RSA_LB_GW 4
+#endif
 
 
 // Lookback words are read and written with agent-scope relaxed atomics

@@ -181,11 +181,15 @@ RSA_HD W lookback_walk(int64_t tile, uint32_t epoch, Load&& load, Backoff&& back
 // reading its kLbGroup tile words, and a group whose last tile has published
 // an inclusive prefix ends the walk.
 // ---------------------------------------------------------------------------
-constexpr int kLbGroup = 8;
+#ifndef RSA_LB_GROUP
+#define RSA_LB_GROUP 8
+#endif
+constexpr int kLbGroup = RSA_LB_GROUP;
+static_assert(kLbGroup >= 2 && kLbGroup <= 63, "the group count field has 6 bits");
 
 template <typename W>
 struct GrpLayout {
-    static constexpr int kCountShift = (int)sizeof(W) * 8 - 4;
+    static constexpr int kCountShift = (int)sizeof(W) * 8 - 6;
     static constexpr W kSumMask = (W(1) << kCountShift) - W(1);
 };
 

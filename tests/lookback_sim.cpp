@@ -114,7 +114,7 @@ int main() {
     if (!rsa::grp_complete(gw) || rsa::grp_sum(gw) != 20480u * rsa::kLbGroup) ++errors;
     uint64_t gw64 = 0;
     for (int i = 0; i < rsa::kLbGroup; ++i) gw64 += rsa::grp_add<uint64_t>(1ull << 40);
-    if (!rsa::grp_complete(gw64) || rsa::grp_sum(gw64) != (8ull << 40)) ++errors;
+    if (!rsa::grp_complete(gw64) || rsa::grp_sum(gw64) != ((uint64_t)rsa::kLbGroup << 40)) ++errors;
     printf("lookback simulation: %d errors\n", errors);
     return errors == 0 ? 0 : 1;
 }
