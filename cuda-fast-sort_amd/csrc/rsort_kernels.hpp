@@ -289,7 +289,6 @@ __device__ __forceinline__ void lds_st(T* p, T v) {
 template <int KB, int K, typename CT>
 __device__ __forceinline__ void rank_wave(const typename KeyTraits<KB>::Reg (&key)[K], uint32_t sh, uint32_t wsel,
                                           uint64_t* M, CT* C, uint32_t lane, uint32_t (&dr)[K]) {
-    const uint64_t lanebit = 1ull << lane;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
         const uint32_t d = key_digit(key[i], sh, wsel);
@@ -301,11 +300,15 @@ __device__ __forceinline__ void rank_wave(const typename KeyTraits<KB>::Reg (&ke
             RSA_COMPILER_FENCE();
             if (lane == 0) lds_st(C + d0, (CT)(c + 64u));
         } else {
-            __hip_atomic_fetch_or(M + d, lanebit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            // each half-wave ors and clears its own 32-bit half of the mask
+            // word (half the LDS bytes of 64-bit ors and clears); the read
+            // is the whole 64-bit word
+            uint32_t* half = (uint32_t*)(M + d) + (lane >> 5);
+            __hip_atomic_fetch_or(half, 1u << (lane & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
             RSA_COMPILER_FENCE();
             const uint64_t m = lds_ld(M + d);
             RSA_COMPILER_FENCE();
-            lds_st(M + d, (uint64_t)0);
+            lds_st(half, 0u);
             RSA_COMPILER_FENCE();
             const uint32_t c = lds_ld(C + d);
             const uint32_t below = mask_below(m);
