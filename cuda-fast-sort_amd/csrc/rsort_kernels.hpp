@@ -664,8 +664,12 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
         return;
     }
     Off* counts = (Off*)a.counts;
+    const uint32_t groups = (gridDim.x + kHistGroup - 1) / kHistGroup;
     // all of a counter's rows are loaded at once (one coherent round trip,
-    // not one per row: at small n this fold is the kernel's critical path)
+    // not one per row: at small n this fold is the kernel's critical path).
+    // With a single group its last arrival holds the totals itself: they go
+    // to LDS and straight into the offsets below, with no device atomics and
+    // no second round trip.
     for (uint32_t i = tid; i < (uint32_t)(P * 256); i += kHistThreads) {
         const uint32_t* col = a.partials + (size_t)grp0 * (P * 256) + i;
         uint32_t v[kHistGroup];
@@ -677,19 +681,22 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
         uint32_t s = 0;
 #pragma unroll
         for (int m = 0; m < kHistGroup; ++m) s += v[m];
-        if (s) atomicAdd(counts + i, (Off)s);
+        if (groups == 1u) s_hist[i] = s;
+        else if (s) atomicAdd(counts + i, (Off)s);
     }
-    const uint32_t groups = (gridDim.x + kHistGroup - 1) / kHistGroup;
     __builtin_amdgcn_s_waitcnt(0);  // device atomics on the counts have completed
     __syncthreads();
-    if (tid == 0) {
-        const uint32_t prev = __hip_atomic_fetch_add(&a.ctrl->hist_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (prev == groups - 1u) ? 1u : 0u;
-    }
-    __syncthreads();
-    if (!s_last) {
-        RSA_TOTAL(20);
-        return;
+    if (groups > 1u) {
+        if (tid == 0) {
+            const uint32_t prev =
+                __hip_atomic_fetch_add(&a.ctrl->hist_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_last = (prev == groups - 1u) ? 1u : 0u;
+        }
+        __syncthreads();
+        if (!s_last) {
+            RSA_TOTAL(20);
+            return;
+        }
     }
 
     // last workgroup: exclusive digit offsets per pass + uniform-pass mask
@@ -699,7 +706,9 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     if (tid < 256) {
 #pragma unroll
         for (int p = 0; p < P; ++p)
-            cp[p] = __hip_atomic_load(counts + p * 256 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            cp[p] = groups == 1u ? (Off)s_hist[p * 256 + tid]
+                                 : __hip_atomic_load(counts + p * 256 + tid, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
