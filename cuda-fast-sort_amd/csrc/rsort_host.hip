@@ -59,6 +59,18 @@ uint32_t pass_kernel_choice() {
     return v;
 }
 
+// Passes of at most this many tiles walk the lookback in segments, one round
+// of loads for up to (threads / 256 - 1) * 4 groups: there every tile starts
+// at once and the walks reach back to the first tiles before those have
+// published inclusive prefixes.  RSORT_AMD_LB_WIDE_TILES overrides it (0 = never).
+uint64_t lb_wide_max_tiles() {
+    static const uint64_t v = [] {
+        const char* e = getenv("RSORT_AMD_LB_WIDE_TILES");
+        return e ? (uint64_t)strtoull(e, nullptr, 10) : (uint64_t)kLbWideMaxTiles;
+    }();
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // Tuning table: per (key width, value width) the pass kernel's tile
 // candidate, the histogram workgroups per CU and the segmented sort's LDS
@@ -164,7 +176,7 @@ struct Plan {
     uint32_t pass_kernel;
     uint32_t tile_cand, hist_bpc;
     uint32_t kind, kb, vb_user, vb_core;
-    bool desc, allow_skip, index_mode, wide;
+    bool desc, allow_skip, index_mode, wide, lb_wide;
     uint32_t passes;
     TileCfg cfg;
     uint64_t tile, num_tiles;
@@ -215,6 +227,7 @@ rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
     p->tile = (uint64_t)p->cfg.threads * p->cfg.items;
     p->num_tiles = n ? (n + p->tile - 1) / p->tile : 0;
     if (p->num_tiles > 0x7FFFFFFFull) return fail(RSORT_ERROR_UNSUPPORTED, "n too large");
+    p->lb_wide = p->num_tiles <= lb_wide_max_tiles();
 
     const size_t off_sz = p->wide ? 8 : 4;
     size_t off = 0;
@@ -391,6 +404,7 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
     j.vb = p.vb_core;
     j.wide = p.wide;
     j.allow_skip = p.allow_skip;
+    j.lb_wide = p.lb_wide;
     j.tw = make_tw_masks(p.kind, p.kb, p.desc);
     j.keys = keys;
     j.tmp_keys = base + p.off_tmp_keys;

@@ -389,16 +389,14 @@ __device__ __forceinline__ void copy_elems(E* __restrict__ dst, const E* __restr
     for (uint64_t i = done + t0; i < elems; i += stride) dst[i] = src[i];
 }
 
-// Keys (and values) back into the caller's buffers, in one launch; the grid
-// is bounded (the common case, an even executed-pass count, is a no-op).
+// Keys (and values) back into the caller's buffers, in one launch, when the
+// executed pass count is odd; the grid is bounded.
 template <typename KE, typename VE>
 __global__ __launch_bounds__(256) void k_copy_back(const Ctrl* __restrict__ ctrl, uint32_t passes,
                                                    KE* __restrict__ dk, const KE* __restrict__ sk,
                                                    VE* __restrict__ dv, const VE* __restrict__ sv, uint64_t elems,
                                                    uint32_t vec_k, uint32_t vec_v) {
-    const uint32_t pmask = passes >= 32 ? ~0u : ((1u << passes) - 1u);
-    const uint32_t active = ~ctrl->skip_mask & pmask;
-    if ((__builtin_popcount(active) & 1u) == 0u) return;
+    if ((__builtin_popcount(pass_run_mask(ctrl->skip_mask, passes)) & 1u) == 0u) return;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     copy_elems(dk, sk, elems, vec_k != 0u, t0, stride);
@@ -745,6 +743,13 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
 #ifndef RSA_LB_GW
 #define RSA_LB_GW 4
 #endif
+// Where the pass kernel loads its values: 0 after the lookback (overlapping
+// the key scatter), 1 with the keys, 2 after the key staging (overlapping
+// the lookback; measured fastest).  Kernels whose LDS allows two workgroups
+// per CU always use 0: their registers are budgeted for it.
+#ifndef RSA_VAL_AT
+#define RSA_VAL_AT 2
+#endif
 
 
 // Lookback words are read and written with agent-scope relaxed atomics
@@ -833,9 +838,50 @@ __device__ __forceinline__ uint32_t onesweep_begin(OnesweepLds<KB, VB, WIDE, THR
     return L.tile;
 }
 
+// How pass `pass` runs (pass_run_mask): whether it runs, whether it is a plain
+// copy, its executed-pass index e (buffer parity: even reads the caller's
+// buffer), its lookback epoch (parity among the passes that walk the lookback:
+// a copy pass writes no lookback words, so it must not flip the epoch), and
+// whether it is the first or last executed pass (twiddle in / out).
+struct PassRun {
+    bool runs, copy, first, last;
+    uint32_t e, epoch;
+};
+
+__device__ __forceinline__ PassRun pass_run(uint32_t skip_mask, uint32_t passes, uint32_t pass) {
+    const uint32_t run = pass_run_mask(skip_mask, passes);
+    const uint32_t below = (1u << pass) - 1u;
+    PassRun r;
+    r.runs = (run >> pass) & 1u;
+    r.copy = (skip_mask >> pass) & 1u;
+    r.e = (uint32_t)__builtin_popcount(run & below);
+    r.epoch = (uint32_t)__builtin_popcount(run & ~skip_mask & below) & 1u;
+    r.first = r.e == 0u;
+    r.last = (run >> (pass + 1u)) == 0u;
+    return r;
+}
+
+// A pass whose digit is uniform, run only to keep the executed pass count
+// even: the stable sort by a constant digit is the identity, so keys
+// [lo, min(hi, n)) are copied (twiddled in or out when it is the first or last
+// executed pass) and values with them, `stride` threads apart.
+template <int KB, int VB>
+__device__ __forceinline__ void copy_pass_range(const PassIo<KB, VB>& io, const TwMasks& tw, uint64_t lo, uint64_t hi,
+                                                uint64_t n, uint64_t stride) {
+    const uint64_t end = hi < n ? hi : n;
+    for (uint64_t i = lo + threadIdx.x; i < end; i += stride) {
+        auto k = key_to_reg<KB>(io.kin[i]);
+        if (io.first) k = tw_in<KB>(k, tw);
+        if (io.last) k = tw_out<KB>(k, tw);
+        io.kout[i] = key_to_mem<KB>(k);
+        if constexpr (VB > 0) io.vout[i] = io.vin[i];
+    }
+}
+
 // One LSD pass over the whole array: one workgroup per tile, tiles taken in
-// ticket order so every tile's predecessors are already running.
-template <int KB, int VB, bool WIDE, int THREADS, int K>
+// ticket order so every tile's predecessors are already running.  SEGWALK:
+// the lookback walk in one-round segments (passes of few tiles).
+template <int KB, int VB, bool WIDE, int THREADS, int K, bool SEGWALK>
 __global__ __launch_bounds__(THREADS, pass_min_waves(KB, VB, K)) void k_onesweep(PassParams a) {
     using KM = typename KeyTraits<KB>::Mem;
     using KR = typename KeyTraits<KB>::Reg;
@@ -847,21 +893,23 @@ __global__ __launch_bounds__(THREADS, pass_min_waves(KB, VB, K)) void k_onesweep
     constexpr int LB_WMAX = RSA_LB_WMAX;
     __shared__ OnesweepLds<KB, VB, WIDE, THREADS, K> L;
 
-    const uint32_t passes = a.passes;
-    const uint32_t pmask = passes >= 32 ? ~0u : ((1u << passes) - 1u);
-    const uint32_t active = ~a.ctrl->skip_mask & pmask;
-    if (!((active >> a.pass) & 1u)) return;  // uniform digit: pass is the identity
-    const uint32_t e = __builtin_popcount(active & ((1u << a.pass) - 1u));
+    const PassRun pr = pass_run(a.ctrl->skip_mask, a.passes, a.pass);
+    if (!pr.runs) return;  // uniform digit: pass is the identity
     PassIo<KB, VB> io;
-    io.kin = (const KM*)((e & 1u) ? a.keys_b : a.keys_a);
-    io.kout = (KM*)((e & 1u) ? a.keys_a : a.keys_b);
-    io.vin = (const VM*)((e & 1u) ? a.vals_b : a.vals_a);
-    io.vout = (VM*)((e & 1u) ? a.vals_a : a.vals_b);
+    io.kin = (const KM*)((pr.e & 1u) ? a.keys_b : a.keys_a);
+    io.kout = (KM*)((pr.e & 1u) ? a.keys_a : a.keys_b);
+    io.vin = (const VM*)((pr.e & 1u) ? a.vals_b : a.vals_a);
+    io.vout = (VM*)((pr.e & 1u) ? a.vals_a : a.vals_b);
     io.sh = 8u * (a.pass & 3u);
     io.wsel = a.pass >> 2;
-    io.epoch = e & 1u;
-    io.first = (e == 0u) && a.tw.active;
-    io.last = ((active >> (a.pass + 1u)) == 0u) && a.tw.active;
+    io.epoch = pr.epoch;
+    io.first = pr.first && a.tw.active;
+    io.last = pr.last && a.tw.active;
+    if (pr.copy) {
+        copy_pass_range<KB, VB>(io, a.tw, (uint64_t)blockIdx.x * TILE, (uint64_t)(blockIdx.x + 1) * TILE, a.n,
+                                THREADS);
+        return;
+    }
 
     RSA_TIMER_START(a.dbg);
 #ifdef RSA_PHASE_TIMING
@@ -877,6 +925,7 @@ __global__ __launch_bounds__(THREADS, pass_min_waves(KB, VB, K)) void k_onesweep
     Off* lb = (Off*)a.lookback;
     constexpr bool LB_GROUPS = true;
     Off* lbg = (Off*)a.lb_groups + (size_t)a.pass * ((a.num_tiles + kLbGroup - 1) / kLbGroup) * 256u;
+    constexpr bool LB_SEGWALK = SEGWALK;
     const TwMasks& tw = a.tw;
     const uint32_t tile_tid = threadIdx.x;
 #include "rsort_onesweep_tile.inc"

@@ -21,6 +21,7 @@ struct SortJob {
     uint32_t vb;          // value bytes as seen by the kernels: 0,1,2,4,8,16
     bool wide;            // 64-bit offsets and lookback words
     bool allow_skip;      // skip passes whose digit is uniform
+    bool lb_wide;         // one-round segmented lookback walk (passes of few tiles)
     TwMasks tw;
     void* keys;           // caller buffer (in/out)
     void* vals;           // caller buffer (in/out) or index buffer

@@ -7,6 +7,21 @@
 
 namespace rsa {
 
+// One pass of the tile kernel; passes of few tiles take the variant with the
+// segmented lookback walk.
+template <int KB, int VB, bool WIDE, int THREADS, int K>
+void launch_onesweep(const SortJob& j, const PassParams& pp, hipStream_t s) {
+    if constexpr (!WIDE && THREADS >= 512) {
+        if (j.lb_wide) {
+            hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, true>), dim3((uint32_t)j.num_tiles),
+                               dim3(THREADS), 0, s, pp);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, false>), dim3((uint32_t)j.num_tiles), dim3(THREADS), 0,
+                       s, pp);
+}
+
 template <int KB, int VB, bool WIDE>
 hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
@@ -24,7 +39,7 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
                        (uint4*)j.ctrl, clear_units);
     record_event(j, s);
 
-    // 2. histogram of all passes, lookback clear, offsets, skip mask
+    // 2. histogram of all passes, offsets, skip mask
     const uint32_t hg = grid_for(j.n, hist_keys_per_block_iter(KB), j.num_cus, j.hist_bpc ? j.hist_bpc : 1u);
     const uint32_t hgrid = hg < kHistMaxGrid ? hg : kHistMaxGrid;
     HistParams h{};
@@ -72,21 +87,20 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
                                dim3(CC.threads), 0, s, pp);
         } else if (tile_cands_built(KB, VB) && j.tile_cand == 1) {
             constexpr TileCfg C1 = tile_cand(KB, VB, 1);
-            hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, C1.threads, C1.items>), dim3((uint32_t)j.num_tiles),
-                               dim3(C1.threads), 0, s, pp);
+            launch_onesweep<KB, VB, WIDE, C1.threads, C1.items>(j, pp, s);
         } else if (tile_cands_built(KB, VB) && j.tile_cand == 2) {
             constexpr TileCfg C2 = tile_cand(KB, VB, 2);
-            hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, C2.threads, C2.items>), dim3((uint32_t)j.num_tiles),
-                               dim3(C2.threads), 0, s, pp);
+            launch_onesweep<KB, VB, WIDE, C2.threads, C2.items>(j, pp, s);
         } else {
-            hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, C.threads, C.items>), dim3((uint32_t)j.num_tiles),
-                               dim3(C.threads), 0, s, pp);
+            launch_onesweep<KB, VB, WIDE, C.threads, C.items>(j, pp, s);
         }
         record_event(j, s);
     }
 
-    // 4. copy back when the executed pass count is odd (decided on device)
-    if ((P & 1u) || j.allow_skip) {
+    // 4. copy back when the executed pass count is odd (decided on the device;
+    //    with skipped passes the count is made even, so only an odd pass count
+    //    can leave the result in the workspace)
+    if (P & 1u) {
         using VE = std::conditional_t<(VB > 0), VM, uint8_t>;
         const uint64_t bytes = j.n * (uint64_t)(KB > VB ? KB : VB);
         const uint32_t g = grid_for(bytes / 16u + 1u, 256u * 8u, j.num_cus, 4);

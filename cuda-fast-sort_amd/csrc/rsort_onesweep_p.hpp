@@ -92,18 +92,21 @@ __global__ __launch_bounds__(THREADS) void k_onesweep_p(PassParams a) {
     __shared__ uint32_t s_dbg[2];
 #endif
 
-    const uint32_t passes = a.passes;
-    const uint32_t pmask = passes >= 32 ? ~0u : ((1u << passes) - 1u);
-    const uint32_t active = ~a.ctrl->skip_mask & pmask;
-    if (!((active >> a.pass) & 1u)) return;  // uniform digit: pass is the identity
-    const uint32_t e = __builtin_popcount(active & ((1u << a.pass) - 1u));
-    const uint32_t epoch = e & 1u;
-    const bool first = (e == 0u) && a.tw.active;
-    const bool last = ((active >> (a.pass + 1u)) == 0u) && a.tw.active;
+    const PassRun pr = pass_run(a.ctrl->skip_mask, a.passes, a.pass);
+    if (!pr.runs) return;  // uniform digit: pass is the identity
+    const uint32_t e = pr.e;
+    const uint32_t epoch = pr.epoch;
+    const bool first = pr.first && a.tw.active;
+    const bool last = pr.last && a.tw.active;
     const KM* __restrict__ kin = (const KM*)((e & 1u) ? a.keys_b : a.keys_a);
     KM* __restrict__ kout = (KM*)((e & 1u) ? a.keys_a : a.keys_b);
     const VM* __restrict__ vin = (const VM*)((e & 1u) ? a.vals_b : a.vals_a);
     VM* __restrict__ vout = (VM*)((e & 1u) ? a.vals_a : a.vals_b);
+    if (pr.copy) {
+        PassIo<KB, VB> io{kin, kout, vin, vout, 0u, 0u, epoch, first, last};
+        copy_pass_range<KB, VB>(io, a.tw, (uint64_t)blockIdx.x * THREADS, a.n, a.n, (uint64_t)gridDim.x * THREADS);
+        return;
+    }
     const uint32_t nchunks = a.num_tiles;
     const uint64_t n = a.n;
     Off* lb = (Off*)a.lookback;

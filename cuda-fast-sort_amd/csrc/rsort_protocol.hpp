@@ -37,6 +37,20 @@ struct Ctrl {
 };
 static_assert(sizeof(Ctrl) == 256, "control header must stay 256 bytes");
 
+// Passes that run.  A pass whose digit is the same for every key is the
+// identity and is skipped, except that when the passes left are odd in
+// number the lowest skipped pass runs as a plain copy: an even count of
+// executed passes leaves the result in the caller's buffer, with no copy-back
+// launch.  Only an odd count with nothing skipped (one-byte keys) ends in the
+// workspace buffer.  Bit p set: pass p runs.
+RSA_HD uint32_t pass_run_mask(uint32_t skip_mask, uint32_t passes) {
+    const uint32_t pmask = passes >= 32 ? ~0u : ((1u << passes) - 1u);
+    const uint32_t active = ~skip_mask & pmask;
+    const uint32_t idle = pmask & ~active;
+    if ((__builtin_popcount(active) & 1u) && idle) return active | (idle & (0u - idle));
+    return active;
+}
+
 // Control block of the segmented sort (same 256-byte slot, cleared per sort).
 // Categories 0..4 are the LDS bins, 5 the onesweep segments.
 struct SegCtrl {
@@ -246,19 +260,15 @@ RSA_HD bool lookback_range(int64_t hi, int64_t lo, uint32_t epoch, Load&& load, 
     return false;
 }
 
-// Exclusive prefix of `tile` over the earlier tiles with group aggregates:
-// the earlier tiles of its own group at tile level, then whole groups, GW per
-// round (the last tile's word and the group word of each), falling back to
-// the tile words of a group that is not complete yet.  Tile 0 publishes an
-// inclusive prefix and group 0 is complete once its tiles have published, so
-// every walk ends.
+// The grouped walk from a given point: the tile words hi, hi-1, ..., lo
+// (none when hi < lo), then whole groups h, h-1, ..., GW per round (the last
+// tile's word and the group word of each), falling back to the tile words of
+// a group that is not complete yet.  Adds to acc and returns the finished
+// prefix.
 template <typename W, int WIN, int GW, typename LoadTile, typename LoadGroup, typename Backoff>
-RSA_HD W lookback_walk_grouped(int64_t tile, uint32_t epoch, LoadTile&& load_tile, LoadGroup&& load_group,
-                               Backoff&& backoff, uint32_t& steps, uint32_t& spins) {
-    W acc = 0;
-    const int64_t g = tile / kLbGroup;
-    if (lookback_range<W, WIN>(tile - 1, g * kLbGroup, epoch, load_tile, backoff, acc, steps, spins)) return acc;
-    int64_t h = g - 1;  // next group to consume
+RSA_HD W lookback_walk_groups_from(W acc, int64_t hi, int64_t lo, int64_t h, uint32_t epoch, LoadTile&& load_tile,
+                                   LoadGroup&& load_group, Backoff&& backoff, uint32_t& steps, uint32_t& spins) {
+    if (lookback_range<W, WIN>(hi, lo, epoch, load_tile, backoff, acc, steps, spins)) return acc;
     while (h >= 0) {
         W last[GW], grp[GW];
 #pragma unroll
@@ -295,6 +305,105 @@ RSA_HD W lookback_walk_grouped(int64_t tile, uint32_t epoch, LoadTile&& load_til
         }
     }
     return acc;
+}
+
+// Exclusive prefix of `tile` over the earlier tiles with group aggregates:
+// the earlier tiles of its own group at tile level, then whole groups.  Tile
+// 0 publishes an inclusive prefix and group 0 is complete once its tiles have
+// published, so every walk ends.
+template <typename W, int WIN, int GW, typename LoadTile, typename LoadGroup, typename Backoff>
+RSA_HD W lookback_walk_grouped(int64_t tile, uint32_t epoch, LoadTile&& load_tile, LoadGroup&& load_group,
+                               Backoff&& backoff, uint32_t& steps, uint32_t& spins) {
+    const int64_t g = tile / kLbGroup;
+    return lookback_walk_groups_from<W, WIN, GW>((W)0, tile - 1, g * kLbGroup, g - 1, epoch, load_tile, load_group,
+                                                 backoff, steps, spins);
+}
+
+// ---------------------------------------------------------------------------
+// Wide walk (plain sorts with few tiles, where every walk runs back to the
+// first tiles while they are still in flight): the walk of one tile is cut
+// into segments that separate threads scan in one round of loads, nearest
+// first.  Segment 0 is the earlier tiles of the tile's own group; segment
+// s >= 1 is the groups g-1-(s-1)*GW down to g-s*GW.  A segment gives its
+// partial sum and how it ended:
+//   kSegOn    passed through every item;
+//   kSegEnd   met an inclusive prefix, or ran past group 0 (the sum is then
+//             complete);
+//   kSegStop  met a not-ready tile word or an incomplete group at item k,
+//             which is stored as k << 2 (items before it are in the sum).
+// lookback_wide_finish adds the segments in order and resumes the ordinary
+// walk where one stopped, so the prefix is the one the ordinary walk finds.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSegOn = 0u, kSegEnd = 1u, kSegStop = 2u;
+
+template <typename W, typename LoadTile>
+RSA_HD uint32_t lb_segment_tiles(int64_t tile, uint32_t epoch, LoadTile&& load_tile, W& part, uint32_t& steps) {
+    constexpr int N = kLbGroup - 1;  // earlier tiles of a group
+    const int64_t lo = (tile / kLbGroup) * kLbGroup;
+    W word[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) word[k] = (tile - 1 - k) >= lo ? load_tile(tile - 1 - k) : (W)0;
+    part = 0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        if ((tile - 1 - k) < lo) return kSegOn;
+        ++steps;
+        if (!lb_ready(word[k], epoch)) return kSegStop | ((uint32_t)k << 2);
+        part += lb_value(word[k]);
+        if (lb_flag(word[k]) == kLbPrefix) return kSegEnd;
+    }
+    return kSegOn;
+}
+
+template <typename W, int GW, typename LoadTile, typename LoadGroup>
+RSA_HD uint32_t lb_segment_groups(int64_t h, uint32_t epoch, LoadTile&& load_tile, LoadGroup&& load_group, W& part,
+                                  uint32_t& steps) {
+    W last[GW], grp[GW];
+#pragma unroll
+    for (int k = 0; k < GW; ++k) {
+        const bool in = (h - k) >= 0;
+        last[k] = in ? load_tile((h - k) * kLbGroup + (kLbGroup - 1)) : (W)0;
+        grp[k] = in ? load_group(h - k) : (W)0;
+    }
+    part = 0;
+#pragma unroll
+    for (int k = 0; k < GW; ++k) {
+        if ((h - k) < 0) return kSegEnd;
+        steps += 2u;
+        if (lb_ready(last[k], epoch) && lb_flag(last[k]) == kLbPrefix) {
+            part += lb_value(last[k]);
+            return kSegEnd;
+        }
+        if (!grp_complete(grp[k])) return kSegStop | ((uint32_t)k << 2);
+        part += grp_sum(grp[k]);
+    }
+    return kSegOn;
+}
+
+// Prefix of `tile` from the results of segments 0..NSEG-1 (part, st).
+template <typename W, int WIN, int GW, int NSEG, typename LoadTile, typename LoadGroup, typename Backoff>
+RSA_HD W lookback_wide_finish(int64_t tile, uint32_t epoch, const W (&part)[NSEG], const uint32_t (&st)[NSEG],
+                              LoadTile&& load_tile, LoadGroup&& load_group, Backoff&& backoff, uint32_t& steps,
+                              uint32_t& spins) {
+    const int64_t g = tile / kLbGroup;
+    W acc = 0;
+#pragma unroll
+    for (int s = 0; s < NSEG; ++s) {
+        acc += part[s];
+        const uint32_t how = st[s] & 3u;
+        const int64_t k = (int64_t)(st[s] >> 2);
+        if (how == kSegEnd) return acc;
+        if (how == kSegStop) {
+            if (s == 0)  // not-ready tile word tile-1-k of the own group
+                return lookback_walk_groups_from<W, WIN, GW>(acc, tile - 1 - k, g * kLbGroup, g - 1, epoch,
+                                                             load_tile, load_group, backoff, steps, spins);
+            const int64_t hs = g - 1 - (int64_t)(s - 1) * GW - k;  // incomplete group
+            return lookback_walk_groups_from<W, WIN, GW>(acc, hs * kLbGroup + (kLbGroup - 1), hs * kLbGroup, hs - 1,
+                                                         epoch, load_tile, load_group, backoff, steps, spins);
+        }
+    }
+    return lookback_walk_groups_from<W, WIN, GW>(acc, -1, 0, g - 1 - (int64_t)(NSEG - 1) * GW, epoch, load_tile,
+                                                 load_group, backoff, steps, spins);
 }
 
 // ---------------------------------------------------------------------------

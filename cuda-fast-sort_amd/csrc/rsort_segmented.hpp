@@ -471,6 +471,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_onesweep(SegParams a) {
     constexpr int LB_WMAX = RSA_LB_WMAX;
     constexpr bool LB_GROUPS = false;  // each segment heads its own chain
     Off* lbg = nullptr;
+    constexpr bool LB_SEGWALK = false;
     __shared__ OnesweepLds<KB, VB, WIDE, THREADS, K> L;
 
     const uint32_t pass = a.pass;

@@ -20,6 +20,10 @@ constexpr int kHistGroup = 32;
 // in the control header).
 constexpr uint32_t kHistMaxGrid = 32 * kHistGroup;
 
+// Passes of at most this many tiles use the one-round segmented lookback
+// walk (all of their tiles are resident at once on 256 CUs).
+constexpr uint64_t kLbWideMaxTiles = 512;
+
 // Histogram grid for n keys of key_bytes: each workgroup gets at least 4
 // loads of 16 bytes per lane.
 RSA_HD constexpr uint32_t hist_keys_per_block_iter(int key_bytes) {

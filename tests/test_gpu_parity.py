@@ -130,6 +130,22 @@ def test_odd_executed_pass_count(rs, orc):
         check(rs, orc, k.view(np.uint8), index_values(4, n), 0, 4, 4, False)
 
 
+@pytest.mark.parametrize("kind,desc", [(1, False), (2, True), (0, True)])
+def test_uniform_pass_run_as_copy(rs, orc, kind, desc):
+    """Odd count of non-uniform passes: the lowest uniform pass runs as a plain
+    copy (twiddled in when it is the first executed pass, out when the last)
+    so the result lands in the caller's buffer.  Signed, float-descending and
+    unsigned-descending keys; u32 with one varying byte, u64 with three."""
+    n = 150_001
+    r = orc.mt19937(11, n).astype(np.uint64)
+    k32 = ((r & 0x00FF0000) | 0x80123456 if kind else (r & 0x00FF0000) | 0x00123456).astype(np.uint32)
+    check(rs, orc, k32.view(np.uint8), index_values(4, n), kind, 4, 4, desc)
+    if kind != 2:
+        k64 = (r & 0xFF) | ((r & 0xFF00) << 16) | ((r & 0xFF0000) << 32) | np.uint64(0x1200330000440000)
+        check(rs, orc, k64.astype(np.uint64).view(np.uint8), index_values(8, n), kind, 8, 8, desc)
+        check(rs, orc, k64.astype(np.uint64).view(np.uint8), None, kind, 8, 0, desc)
+
+
 def test_reference_init_keys_modes(rs, orc):
     """The reference's own generator (utils.cuh:245-317) in its three modes."""
     n = 100_000
