@@ -7,10 +7,11 @@ For each tuned (key bytes, value bytes) it times, with the library's own
 per-stage device events (rsort_sort_profiled):
   * every pass-kernel tile candidate            -> fastest mean pass time
   * histogram workgroups per CU in 1..4         -> fastest histogram
-and, with device events around the whole rsort_sort call at 2^20, 2^22,
-2^24 and 2^26 keys:
+and, with device events around the whole rsort_sort call at 2^20 .. 2^24
+and 2^26 keys:
   * every tile candidate again                  -> the small-sort candidate
-    and the size below which it is used (small_tile_candidate, small_log2n)
+    and the size below which it is used (small_tile_candidate, small_log2n),
+    scored by the summed time relative to the fastest candidate per size
 and, with device events around rsort_segmented_sort on a batch of
 lognormal-sized segments (median 2048, sigma 0.7356, clipped [32, 2^22]):
   * the LDS threshold (each LDS bin capacity)   -> fastest segmented sort.
@@ -33,6 +34,8 @@ sys.path.insert(0, HERE)
 
 # (key bytes, value bytes) tuned by default: the widths with tile candidates
 TUNED = [(4, 0), (4, 4), (8, 0), (8, 8)]
+# pass-kernel tile candidates (kTileCands in csrc/rsort_tuning.hpp)
+TILE_CANDIDATES = 4
 
 
 def lognormal_sizes(count, seed=5):
@@ -91,19 +94,27 @@ def time_whole(rs, torch, kb, vb, n, reps):
     return ms[len(ms) // 2]
 
 
-def small_choice(best_by_size, large_cand):
-    """(small candidate, small_log2n) from the best candidate per measured
-    log2 size: the sizes from the smallest up whose best differs from the
-    large-sort candidate, up to the first that agrees."""
-    sizes = sorted(best_by_size)
-    below = []
-    for s in sizes:
-        if best_by_size[s] == large_cand:
-            break
-        below.append(s)
-    if not below:
-        return 0, 0
-    return best_by_size[below[-1]], below[-1] + 1
+def small_choice(small_ms, large_cand):
+    """(small candidate, small_log2n): sorts below 2^small_log2n use the small
+    candidate, larger ones large_cand.  Every candidate and every threshold
+    just above a measured size is scored by the sum over the measured sizes
+    of its time relative to the fastest candidate there; the lowest score
+    wins, (0, 0) -- large_cand everywhere -- on ties."""
+    sizes = sorted(small_ms)
+    best = {s: min(small_ms[s].values()) for s in sizes}
+
+    def score(cand, log2n):
+        return sum(small_ms[s][cand if s < log2n else large_cand] / best[s] for s in sizes)
+
+    choice, top = (0, 0), score(large_cand, 0)
+    for cand in small_ms[sizes[0]]:
+        if cand == large_cand:
+            continue
+        for s in sizes:
+            sc = score(cand, s + 1)
+            if sc < top - 1e-9:
+                choice, top = (cand, s + 1), sc
+    return choice
 
 
 def time_segmented(rs, torch, kb, vb, sizes, reps):
@@ -155,7 +166,7 @@ def main():
         caps = list(rs.segmented_plan(rs.make_desc(0, kb, vb), 1 << 20, 8, 1).bin_capacity)
         # tile candidates (histogram setting fixed at its default)
         tile_ms = {}
-        for c in range(3):
+        for c in range(TILE_CANDIDATES):
             rs.tuning_set(kb, vb, c, base.hist_blocks_per_cu, base.lds_threshold)
             tile_ms[c], _ = time_flat(rs, torch, kb, vb, n, a.reps)
         best_tile = min(tile_ms, key=tile_ms.get)
@@ -172,15 +183,14 @@ def main():
             seg_ms[cap] = time_segmented(rs, torch, kb, vb, sizes, a.reps)
         best_cap = min(seg_ms, key=seg_ms.get)
         # tile candidate of small sorts
-        small_ms, best_by_size = {}, {}
-        for l2 in [s for s in (20, 22, 24, 26) if s < a.log2n]:
+        small_ms = {}
+        for l2 in [s for s in (20, 21, 22, 23, 24, 26) if s < a.log2n]:
             row = {}
-            for c in range(3):
+            for c in range(TILE_CANDIDATES):
                 rs.tuning_set(kb, vb, c, best_bpc, best_cap)
                 row[c] = time_whole(rs, torch, kb, vb, 1 << l2, a.reps + 4)
             small_ms[l2] = row
-            best_by_size[l2] = min(row, key=row.get)
-        small_cand, small_log2n = small_choice(best_by_size, best_tile)
+        small_cand, small_log2n = small_choice(small_ms, best_tile)
         rs.tuning_set(kb, vb, best_tile, best_bpc, best_cap, small_cand, small_log2n)
         lines.append(f"{kb} {vb} {best_tile} {best_bpc} {best_cap} {small_cand} {small_log2n}")
         report.append({"key_bytes": kb, "value_bytes": vb, "pass_ms_by_tile_candidate": tile_ms,

@@ -91,6 +91,9 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
         } else if (tile_cands_built(KB, VB) && j.tile_cand == 2) {
             constexpr TileCfg C2 = tile_cand(KB, VB, 2);
             launch_onesweep<KB, VB, WIDE, C2.threads, C2.items>(j, pp, s);
+        } else if (tile_cands_built(KB, VB) && j.tile_cand == 3) {
+            constexpr TileCfg C3 = tile_cand(KB, VB, 3);
+            launch_onesweep<KB, VB, WIDE, C3.threads, C3.items>(j, pp, s);
         } else {
             launch_onesweep<KB, VB, WIDE, C.threads, C.items>(j, pp, s);
         }

@@ -135,10 +135,11 @@ RSA_HD constexpr int pass_min_waves(int key_bytes, int val_bytes, int items) {
 }
 
 // Tile candidates of the pass kernel the autotuner chooses from.  Candidate
-// 0 is tile_cfg; 1 and 2 (fewer items per thread; half the threads) are
-// built for the widths sorted most (4- and 8-byte keys, keys only or with
-// values of the key's width) and fall back to candidate 0 elsewhere.
-constexpr int kTileCands = 3;
+// 0 is tile_cfg; 1, 2 and 3 (four items fewer per thread; half the threads;
+// half the items) are built for the widths sorted most (4- and 8-byte keys,
+// keys only or with values of the key's width) and fall back to candidate 0
+// elsewhere.
+constexpr int kTileCands = 4;
 
 RSA_HD constexpr bool tile_cands_built(int key_bytes, int val_bytes) {
     return (key_bytes == 4 || key_bytes == 8) && (val_bytes == 0 || val_bytes == key_bytes);
@@ -148,7 +149,8 @@ RSA_HD constexpr TileCfg tile_cand(int key_bytes, int val_bytes, int cand) {
     const TileCfg d = tile_cfg(key_bytes, val_bytes);
     if (!tile_cands_built(key_bytes, val_bytes) || cand <= 0 || cand >= kTileCands) return d;
     if (cand == 1) return TileCfg{d.threads, d.items - 4u};
-    return TileCfg{d.threads / 2u, d.items};
+    if (cand == 2) return TileCfg{d.threads / 2u, d.items};
+    return TileCfg{d.threads, d.items / 2u};
 }
 
 // ---------------------------------------------------------------------------

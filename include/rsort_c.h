@@ -197,9 +197,10 @@ rsort_status rsort_digit_counts(const rsort_desc* desc, const void* keys, uint64
  * Tuning table (what the autotuner chooses), per key width x value width
  * (value widths other than 0,1,2,4,8,16 use the 4-byte entry):
  *   tile_candidate      pass-kernel tile: 0 = the built-in geometry, 1 = four
- *                       items fewer per thread, 2 = half the threads (1 and 2
- *                       exist for 4- and 8-byte keys, keys only or with values
- *                       of the key's width; elsewhere they mean 0)
+ *                       items fewer per thread, 2 = half the threads, 3 = half
+ *                       the items per thread (1-3 exist for 4- and 8-byte
+ *                       keys, keys only or with values of the key's width;
+ *                       elsewhere they mean 0)
  *   hist_blocks_per_cu  histogram workgroups per CU, 1..8
  *   lds_threshold       segmented sort: longest segment sorted in LDS (LDS
  *                       bins larger than it are not used), <= bin_capacity[4]

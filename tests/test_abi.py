@@ -187,11 +187,13 @@ def test_tuning_defaults_set_and_plan(rs, tmp_path):
     rs.tuning_load("")  # built-in defaults only
     t = rs.tuning_get(4, 4)
     assert (t.tile_candidate, t.lds_threshold) == (0, 16384) and 1 <= t.hist_blocks_per_cu <= 8
-    assert (t.small_tile_candidate, t.small_log2n) == (1, 27)
+    assert (t.small_tile_candidate, t.small_log2n) == (3, 25)
     d = rs.make_desc(0, 4, 4)
-    # below 2^27 keys the small-sort candidate (1024 x 16) is used
-    small = rs.plan(d, (1 << 27) - 1)
-    assert (small.threads, small.items_per_thread) == (1024, 16)
+    # below 2^25 keys the small-sort candidate (1024 x 10) is used
+    small = rs.plan(d, (1 << 25) - 1)
+    assert (small.threads, small.items_per_thread) == (1024, 10)
+    base = rs.plan(d, 1 << 25)
+    assert (base.threads, base.items_per_thread) == (1024, 20)
     base = rs.plan(d, 1 << 27)
     assert (base.threads, base.items_per_thread) == (1024, 20)
     try:
@@ -202,6 +204,9 @@ def test_tuning_defaults_set_and_plan(rs, tmp_path):
         rs.tuning_set(4, 4, 2, 2, 8192)
         p2 = rs.plan(d, 1 << 20)
         assert (p2.threads, p2.items_per_thread) == (512, 20)
+        rs.tuning_set(4, 4, 3, 2, 8192)
+        p3 = rs.plan(d, 1 << 20)
+        assert (p3.threads, p3.items_per_thread) == (1024, 10)
         # the LDS threshold moves segments of 8193..16384 keys to the onesweep path
         sp = rs.segmented_plan(d, 1 << 22, 4, 1000)
         assert sp.max_long_segments == (1 << 22) // 8193
@@ -222,7 +227,7 @@ def test_tuning_defaults_set_and_plan(rs, tmp_path):
         assert rs.tuning_load(str(tmp_path / "missing.txt")) == -1
     finally:
         rs.tuning_load(None)
-    assert rs.plan(d, 1 << 20).items_per_thread == 16 and rs.plan(d, 1 << 28).items_per_thread == 20
+    assert rs.plan(d, 1 << 20).items_per_thread == 10 and rs.plan(d, 1 << 28).items_per_thread == 20
 
 
 def test_persisted_table_is_read_at_first_use(rs):
@@ -240,8 +245,8 @@ def test_persisted_table_is_read_at_first_use(rs):
         assert (t.small_tile_candidate, t.small_log2n) == (stc, sl)
 
 
-@pytest.mark.parametrize("args", [(4, 4, 3, 1, 0), (4, 4, 0, 0, 0), (4, 4, 0, 9, 0), (4, 4, 0, 1, 16385),
-                                  (3, 4, 0, 1, 0), (4, 12, 0, 1, 0), (4, 4, 0, 1, 0, 3, 20), (4, 4, 0, 1, 0, 1, 41)])
+@pytest.mark.parametrize("args", [(4, 4, 4, 1, 0), (4, 4, 0, 0, 0), (4, 4, 0, 9, 0), (4, 4, 0, 1, 16385),
+                                  (3, 4, 0, 1, 0), (4, 12, 0, 1, 0), (4, 4, 0, 1, 0, 4, 20), (4, 4, 0, 1, 0, 1, 41)])
 def test_tuning_rejects_out_of_range(rs, args):
     with pytest.raises(rs.RsortError):
         rs.tuning_set(*args)
@@ -268,3 +273,30 @@ def test_sort_records_query_and_arguments(rs):
     st = rs.lib().rsort_sort_records(ctypes.byref(d), ctypes.c_void_p(0x1000), 1000, 12, 0, ctypes.c_void_p(0x1000),
                                      ctypes.byref(t), None)
     assert st == 3
+
+
+# ---------------------------------------------------------------------------
+# Autotuner: choice of the small-sort tile candidate (host logic only)
+# ---------------------------------------------------------------------------
+def test_autotune_small_choice():
+    import importlib.util
+    import os
+
+    spec = importlib.util.spec_from_file_location(
+        "autotune", os.path.join(os.path.dirname(__file__), "..", "cuda-fast-sort_amd", "autotune.py"))
+    at = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(at)
+    # whole-sort microseconds by log2 n and candidate, u32 keys and u32/u32 on MI355X
+    keys = {20: {0: 85.7, 1: 76.3, 2: 79.9, 3: 72.6}, 21: {0: 98.7, 1: 93.3, 2: 97.7, 3: 91.2},
+            22: {0: 112.8, 1: 121.0, 2: 125.5, 3: 121.5}, 23: {0: 149.1, 1: 163.2, 2: 179.6, 3: 170.4},
+            24: {0: 229.8, 1: 239.2, 2: 246.8, 3: 255.2}}
+    assert at.small_choice(keys, 0) == (3, 22)
+    pairs = {20: {0: 109.8, 1: 100.9, 2: 99.8, 3: 87.0}, 22: {0: 137.0, 1: 119.9, 2: 133.7, 3: 132.6},
+             24: {0: 341.8, 1: 314.0, 2: 342.1, 3: 329.1}, 26: {0: 1190.0, 1: 1230.0, 2: 1260.0, 3: 1300.0}}
+    # cand 3 below 2^25 sums to 4.154 relative, cand 1 below 2^25 to 4.160
+    assert at.small_choice(pairs, 0) == (3, 25)
+    pairs[20][3] = 95.0
+    assert at.small_choice(pairs, 0) == (1, 25)
+    # the large-sort candidate fastest everywhere: no small candidate
+    flat = {20: {0: 1.0, 1: 2.0}, 22: {0: 1.0, 1: 1.5}}
+    assert at.small_choice(flat, 0) == (0, 0)

@@ -121,7 +121,8 @@ def test_all_keys_equal(rs, orc, kb):
 
 def test_odd_executed_pass_count(rs, orc):
     """u32 keys whose bytes 1 and 3 vary and bytes 0 and 2 are constant:
-    two passes execute (even); bytes 0,1,2 constant: one pass (odd -> copy back)."""
+    two passes execute (even); bytes 0,1,2 constant: one non-uniform pass
+    (odd: pass 0 runs as a copy to make the count even)."""
     n = 200_000
     r = orc.mt19937(3, n)
     k_even = ((r & 0xFF00FF00) | 0x00550055).astype(np.uint32)
@@ -230,13 +231,18 @@ def test_python_api_mirror(rs, orc):
 
 
 @pytest.mark.parametrize("kb,vb", [(4, 0), (4, 4), (8, 0), (8, 8)])
-@pytest.mark.parametrize("cand", [1, 2])
+@pytest.mark.parametrize("cand", [0, 1, 2, 3])
 def test_tile_candidates(rs, orc, kb, vb, cand):
-    """Every pass-kernel tile candidate the autotuner may choose."""
+    """Every pass-kernel tile candidate the autotuner may choose, at every
+    size (passes of at most 512 tiles use the segmented lookback walk; the
+    5M-key u32 case has more tiles and uses the ordinary walk)."""
     try:
         t = rs.tuning_get(kb, vb)
         rs.tuning_set(kb, vb, cand, t.hist_blocks_per_cu, t.lds_threshold)
-        for n, desc in ((1000, False), (70001, True), (300007, False)):
+        cases = [(1000, False), (70001, True), (300007, False)]
+        if kb == 4:
+            cases.append((5_000_011, False))
+        for n, desc in cases:
             keys = random_keys(0, kb, n, seed=cand * 100 + n)
             vals = index_values(vb, n) if vb else None
             check(rs, orc, keys, vals, 0, kb, vb, desc)
