@@ -71,6 +71,17 @@ uint64_t lb_wide_max_tiles() {
     return v;
 }
 
+// Histogram grids of more than one fold group and at most this many
+// workgroups add their counts with device atomics directly instead of the
+// two-level fold.  RSORT_AMD_HIST_DIRECT overrides it (0 = never).
+uint32_t hist_direct_max() {
+    static const uint32_t v = [] {
+        const char* e = getenv("RSORT_AMD_HIST_DIRECT");
+        return e ? (uint32_t)strtoul(e, nullptr, 10) : kHistDirectMaxGrid;
+    }();
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // Tuning table: per (key width, value width) the pass kernel's tile
 // candidate, the histogram workgroups per CU and the segmented sort's LDS
@@ -406,6 +417,7 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
     j.wide = p.wide;
     j.allow_skip = p.allow_skip;
     j.lb_wide = p.lb_wide;
+    j.hist_direct_max = hist_direct_max();
     j.tw = make_tw_masks(p.kind, p.kb, p.desc);
     j.keys = keys;
     j.tmp_keys = base + p.off_tmp_keys;

@@ -334,6 +334,7 @@ struct HistParams {
     uint64_t head;         // keys before the first 16-byte aligned group
     uint32_t passes;
     uint32_t allow_skip;
+    uint32_t direct;       // fold by device atomics from every workgroup (small grids)
     TwMasks tw;
     unsigned long long* dbg;  // phase-timing slots (RSA_PHASE_TIMING builds), else null
 };
@@ -628,72 +629,99 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     __syncthreads();
     RSA_MARK(16);
 
-    // Two-level fold: every workgroup stores its counts; the last arrival of
-    // each group of kHistGroup workgroups sums the group's rows and adds them
-    // to the global counts (kHistGroup x fewer device atomics); the last group
-    // to finish finalises.
-    {
-        uint32_t* row = a.partials + (size_t)blockIdx.x * (P * 256);
+    Off* counts = (Off*)a.counts;
+    const uint32_t groups = (gridDim.x + kHistGroup - 1) / kHistGroup;
+    bool from_lds = false;  // the finalising workgroup holds the totals in LDS
+    if (a.direct) {
+        // Small grids: every workgroup adds its counts to the global counts
+        // with device atomics, and the last arrival finalises (three coherent
+        // round trips fewer than the two-level fold below).
         for (uint32_t i = tid; i < (uint32_t)(P * 256); i += kHistThreads) {
             uint32_t s = 0;
 #pragma unroll
             for (int c = 0; c < COPIES; ++c) s += s_hist[c * P * 256 + i];
-            __hip_atomic_store(row + i, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (s) atomicAdd(counts + i, (Off)s);
         }
-    }
-    const uint32_t group = blockIdx.x / kHistGroup;
-    const uint32_t grp0 = group * kHistGroup;
-    const uint32_t gsize = (gridDim.x - grp0) < (uint32_t)kHistGroup ? (gridDim.x - grp0) : (uint32_t)kHistGroup;
-    // The rows were written with coherent (write-through) atomic stores, so
-    // waiting for their completion orders them before the arrival count; an
-    // agent-scope release here would write back the whole XCD L2 and, with
-    // every workgroup of the XCD arriving together, serialise on it.
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (tid == 0) {
-        const uint32_t prev =
-            __hip_atomic_fetch_add(&a.ctrl->group_done[group], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (prev == gsize - 1u) ? 1u : 0u;
-    }
-    __syncthreads();
-    RSA_MARK(17);
-    if (!s_last) {
-        RSA_TOTAL(20);
-        return;
-    }
-    Off* counts = (Off*)a.counts;
-    const uint32_t groups = (gridDim.x + kHistGroup - 1) / kHistGroup;
-    // all of a counter's rows are loaded at once (one coherent round trip,
-    // not one per row: at small n this fold is the kernel's critical path).
-    // With a single group its last arrival holds the totals itself: they go
-    // to LDS and straight into the offsets below, with no device atomics and
-    // no second round trip.
-    for (uint32_t i = tid; i < (uint32_t)(P * 256); i += kHistThreads) {
-        const uint32_t* col = a.partials + (size_t)grp0 * (P * 256) + i;
-        uint32_t v[kHistGroup];
-#pragma unroll
-        for (int m = 0; m < kHistGroup; ++m)
-            v[m] = (uint32_t)m < gsize
-                       ? __hip_atomic_load(col + (size_t)m * (P * 256), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                       : 0u;
-        uint32_t s = 0;
-#pragma unroll
-        for (int m = 0; m < kHistGroup; ++m) s += v[m];
-        if (groups == 1u) s_hist[i] = s;
-        else if (s) atomicAdd(counts + i, (Off)s);
-    }
-    __builtin_amdgcn_s_waitcnt(0);  // device atomics on the counts have completed
-    __syncthreads();
-    if (groups > 1u) {
+        __builtin_amdgcn_s_waitcnt(0);  // device atomics on the counts have completed
+        __syncthreads();
         if (tid == 0) {
             const uint32_t prev =
                 __hip_atomic_fetch_add(&a.ctrl->hist_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_last = (prev == groups - 1u) ? 1u : 0u;
+            s_last = (prev == gridDim.x - 1u) ? 1u : 0u;
         }
         __syncthreads();
+        RSA_MARK(17);
         if (!s_last) {
             RSA_TOTAL(20);
             return;
+        }
+    } else {
+        // Two-level fold: every workgroup stores its counts; the last arrival of
+        // each group of kHistGroup workgroups sums the group's rows and adds them
+        // to the global counts (kHistGroup x fewer device atomics); the last group
+        // to finish finalises.
+        {
+            uint32_t* row = a.partials + (size_t)blockIdx.x * (P * 256);
+            for (uint32_t i = tid; i < (uint32_t)(P * 256); i += kHistThreads) {
+                uint32_t s = 0;
+#pragma unroll
+                for (int c = 0; c < COPIES; ++c) s += s_hist[c * P * 256 + i];
+                __hip_atomic_store(row + i, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        const uint32_t group = blockIdx.x / kHistGroup;
+        const uint32_t grp0 = group * kHistGroup;
+        const uint32_t gsize = (gridDim.x - grp0) < (uint32_t)kHistGroup ? (gridDim.x - grp0) : (uint32_t)kHistGroup;
+        // The rows were written with coherent (write-through) atomic stores, so
+        // waiting for their completion orders them before the arrival count; an
+        // agent-scope release here would write back the whole XCD L2 and, with
+        // every workgroup of the XCD arriving together, serialise on it.
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) {
+            const uint32_t prev =
+                __hip_atomic_fetch_add(&a.ctrl->group_done[group], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_last = (prev == gsize - 1u) ? 1u : 0u;
+        }
+        __syncthreads();
+        RSA_MARK(17);
+        if (!s_last) {
+            RSA_TOTAL(20);
+            return;
+        }
+        // all of a counter's rows are loaded at once (one coherent round trip,
+        // not one per row: at small n this fold is the kernel's critical path).
+        // With a single group its last arrival holds the totals itself: they go
+        // to LDS and straight into the offsets below, with no device atomics and
+        // no second round trip.
+        for (uint32_t i = tid; i < (uint32_t)(P * 256); i += kHistThreads) {
+            const uint32_t* col = a.partials + (size_t)grp0 * (P * 256) + i;
+            uint32_t v[kHistGroup];
+#pragma unroll
+            for (int m = 0; m < kHistGroup; ++m)
+                v[m] = (uint32_t)m < gsize
+                           ? __hip_atomic_load(col + (size_t)m * (P * 256), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : 0u;
+            uint32_t s = 0;
+#pragma unroll
+            for (int m = 0; m < kHistGroup; ++m) s += v[m];
+            if (groups == 1u) s_hist[i] = s;
+            else if (s) atomicAdd(counts + i, (Off)s);
+        }
+        from_lds = groups == 1u;
+        __builtin_amdgcn_s_waitcnt(0);  // device atomics on the counts have completed
+        __syncthreads();
+        if (groups > 1u) {
+            if (tid == 0) {
+                const uint32_t prev =
+                    __hip_atomic_fetch_add(&a.ctrl->hist_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_last = (prev == groups - 1u) ? 1u : 0u;
+            }
+            __syncthreads();
+            if (!s_last) {
+                RSA_TOTAL(20);
+                return;
+            }
         }
     }
 
@@ -704,9 +732,8 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     if (tid < 256) {
 #pragma unroll
         for (int p = 0; p < P; ++p)
-            cp[p] = groups == 1u ? (Off)s_hist[p * 256 + tid]
-                                 : __hip_atomic_load(counts + p * 256 + tid, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
+            cp[p] = from_lds ? (Off)s_hist[p * 256 + tid]
+                             : __hip_atomic_load(counts + p * 256 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {

@@ -57,6 +57,7 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     }
     h.passes = P;
     h.allow_skip = j.allow_skip ? 1u : 0u;
+    h.direct = (hgrid > (uint32_t)kHistGroup && hgrid <= j.hist_direct_max) ? 1u : 0u;
     h.tw = j.tw;
     h.dbg = j.dbg;
     hipLaunchKernelGGL((k_histogram<KB, WIDE>), dim3(hgrid), dim3(kHistThreads), 0, s, h);

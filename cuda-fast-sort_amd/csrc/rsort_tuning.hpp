@@ -19,6 +19,10 @@ constexpr int kHistGroup = 32;
 // Upper bound of the histogram grid (32 groups, the arrival counters that fit
 // in the control header).
 constexpr uint32_t kHistMaxGrid = 32 * kHistGroup;
+// Histogram grids up to this size fold by device atomics from every
+// workgroup (fewer coherent round trips; the two-level fold above it keeps
+// the device atomics few).
+constexpr uint32_t kHistDirectMaxGrid = 256;
 
 // Passes of at most this many tiles use the one-round segmented lookback
 // walk (all of their tiles are resident at once on 256 CUs).
