@@ -9,14 +9,16 @@ per-stage device events (rsort_sort_profiled):
   * histogram workgroups per CU in 1..4         -> fastest histogram
 and, with device events around the whole rsort_sort call at 2^20 .. 2^24
 and 2^26 keys:
-  * every tile candidate again                  -> the small-sort candidate
-    and the size below which it is used (small_tile_candidate, small_log2n),
-    scored by the summed time relative to the fastest candidate per size
+  * every tile candidate again                  -> up to two small-size tiers
+    (tiny_tile_candidate below 2^tiny_log2n, small_tile_candidate below
+    2^small_log2n), scored by the summed time relative to the fastest
+    candidate per size
 and, with device events around rsort_segmented_sort on a batch of
 lognormal-sized segments (median 2048, sigma 0.7356, clipped [32, 2^22]):
   * the LDS threshold (each LDS bin capacity)   -> fastest segmented sort.
 The table (one line per entry: key_bytes value_bytes tile_candidate
-hist_blocks_per_cu lds_threshold small_tile_candidate small_log2n) goes to --out, by default
+hist_blocks_per_cu lds_threshold small_tile_candidate small_log2n
+tiny_tile_candidate tiny_log2n) goes to --out, by default
 tuning_gfx950.txt next to librsort_amd.so, and is printed.  Measurements are
 on synthetic uniform keys; the reference has no tuner (its geometry is a
 fixed sm_86 table, include/rsort/detail/tuning.cuh:138-229).
@@ -95,25 +97,40 @@ def time_whole(rs, torch, kb, vb, n, reps):
 
 
 def small_choice(small_ms, large_cand):
-    """(small candidate, small_log2n): sorts below 2^small_log2n use the small
-    candidate, larger ones large_cand.  Every candidate and every threshold
-    just above a measured size is scored by the sum over the measured sizes
-    of its time relative to the fastest candidate there; the lowest score
-    wins, (0, 0) -- large_cand everywhere -- on ties."""
+    """Small-size tiers (tiny_cand, tiny_log2n, small_cand, small_log2n):
+    sorts below 2^tiny_log2n use tiny_cand, those below 2^small_log2n
+    small_cand, larger ones large_cand; a log2n of 0 switches a tier off.
+    Every choice of candidates and of thresholds just above measured sizes is
+    scored by the sum over the measured sizes of its time relative to the
+    fastest candidate there.  The lowest score wins; on ties the choice with
+    fewer tiers (none, then the small tier alone)."""
     sizes = sorted(small_ms)
     best = {s: min(small_ms[s].values()) for s in sizes}
+    cands = list(small_ms[sizes[0]])
+    limits = [s + 1 for s in sizes]
 
-    def score(cand, log2n):
-        return sum(small_ms[s][cand if s < log2n else large_cand] / best[s] for s in sizes)
-
-    choice, top = (0, 0), score(large_cand, 0)
-    for cand in small_ms[sizes[0]]:
-        if cand == large_cand:
-            continue
+    def score(tc, tl, sc, sl):
+        total = 0.0
         for s in sizes:
-            sc = score(cand, s + 1)
-            if sc < top - 1e-9:
-                choice, top = (cand, s + 1), sc
+            c = tc if (tl and s < tl) else sc if (sl and s < sl) else large_cand
+            total += small_ms[s][c] / best[s]
+        return total
+
+    choice, top = (0, 0, 0, 0), score(0, 0, large_cand, 0)
+    for sc in cands:  # one tier
+        for sl in limits:
+            v = score(0, 0, sc, sl)
+            if v < top - 1e-9:
+                choice, top = (0, 0, sc, sl), v
+    for tc in cands:  # two tiers
+        for tl in limits:
+            for sc in cands:
+                for sl in limits:
+                    if sl <= tl or sc == tc:
+                        continue
+                    v = score(tc, tl, sc, sl)
+                    if v < top - 1e-9:
+                        choice, top = (tc, tl, sc, sl), v
     return choice
 
 
@@ -190,21 +207,23 @@ def main():
                 rs.tuning_set(kb, vb, c, best_bpc, best_cap)
                 row[c] = time_whole(rs, torch, kb, vb, 1 << l2, a.reps + 4)
             small_ms[l2] = row
-        small_cand, small_log2n = small_choice(small_ms, best_tile)
-        rs.tuning_set(kb, vb, best_tile, best_bpc, best_cap, small_cand, small_log2n)
-        lines.append(f"{kb} {vb} {best_tile} {best_bpc} {best_cap} {small_cand} {small_log2n}")
+        tiny_cand, tiny_log2n, small_cand, small_log2n = small_choice(small_ms, best_tile)
+        rs.tuning_set(kb, vb, best_tile, best_bpc, best_cap, small_cand, small_log2n, tiny_cand, tiny_log2n)
+        lines.append(f"{kb} {vb} {best_tile} {best_bpc} {best_cap} {small_cand} {small_log2n} {tiny_cand} {tiny_log2n}")
         report.append({"key_bytes": kb, "value_bytes": vb, "pass_ms_by_tile_candidate": tile_ms,
                        "histogram_ms_by_blocks_per_cu": hist_ms, "segmented_ms_by_lds_threshold": seg_ms,
                        "sort_ms_by_log2n_and_candidate": small_ms,
                        "chosen": {"tile_candidate": best_tile, "hist_blocks_per_cu": best_bpc,
                                   "lds_threshold": best_cap, "small_tile_candidate": small_cand,
-                                  "small_log2n": small_log2n}})
+                                  "small_log2n": small_log2n, "tiny_tile_candidate": tiny_cand,
+                                  "tiny_log2n": tiny_log2n}})
         print(json.dumps(report[-1]), flush=True)
     props = torch.cuda.get_device_properties(0)
     header = [f"# rsort tuning table for {getattr(props, 'gcnArchName', 'gfx950')} ({props.name}), written by autotune.py",
               f"# {time.strftime('%Y-%m-%d')}: flat sorts of 2^{a.log2n} uniform keys, segmented sorts of "
               f"{a.segments} lognormal segments, mean of {a.reps} runs each",
-              "# key_bytes value_bytes tile_candidate hist_blocks_per_cu lds_threshold small_tile_candidate small_log2n"]
+              "# key_bytes value_bytes tile_candidate hist_blocks_per_cu lds_threshold small_tile_candidate small_log2n "
+              "tiny_tile_candidate tiny_log2n"]
     text = "\n".join(header + lines) + "\n"
     with open(a.out, "w") as f:
         f.write(text)

@@ -87,7 +87,8 @@ uint32_t hist_direct_max() {
 // candidate, the histogram workgroups per CU and the segmented sort's LDS
 // threshold, plus the tile candidate of small sorts.  Built-in defaults,
 // overridden by a text table ("key_bytes value_bytes tile_candidate
-// hist_blocks_per_cu lds_threshold [small_tile_candidate small_log2n]" per
+// hist_blocks_per_cu lds_threshold [small_tile_candidate small_log2n
+// [tiny_tile_candidate tiny_log2n]]" per
 // line, '#' comments) read once from $RSORT_AMD_TUNING_FILE or tuning_gfx950.txt next to
 // this library, and by rsort_tuning_set.  The table is written by the
 // autotuner (autotune.py).
@@ -95,6 +96,7 @@ uint32_t hist_direct_max() {
 struct TuneEntry {
     uint32_t tile_cand, hist_bpc, lds_threshold;
     uint32_t small_tile_cand, small_log2n;  // candidate of sorts below 2^small_log2n keys
+    uint32_t tiny_tile_cand, tiny_log2n;    // candidate of sorts below 2^tiny_log2n keys (checked first)
 };
 
 int kb_index(uint32_t kb) { return kb == 1 ? 0 : kb == 2 ? 1 : kb == 4 ? 2 : kb == 8 ? 3 : kb == 16 ? 4 : -1; }
@@ -109,12 +111,13 @@ TuneEntry g_tune[5][6];
 bool g_tune_ready = false;
 
 // Measured on MI355X (whole-call time, uniform keys): u32 keys sort faster
-// with 1024 x 8 tiles below 2^22 keys, u32/u32 with 1024 x 10 tiles below
-// 2^25, u64/u64 with 1024 x 5 tiles below 2^21.
+// with 1024 x 8 tiles below 2^22 keys; u32/u32 with 1024 x 10 tiles below
+// 2^22 and 1024 x 16 tiles (two workgroups per CU) below 2^25; u64/u64 with
+// 1024 x 5 tiles below 2^21.
 TuneEntry tune_default(uint32_t kb, uint32_t vb) {
-    TuneEntry t{0u, hist_blocks_per_cu((int)kb), seg_cap((int)kb, (int)vb, kSegBins - 1), 0u, 0u};
+    TuneEntry t{0u, hist_blocks_per_cu((int)kb), seg_cap((int)kb, (int)vb, kSegBins - 1), 0u, 0u, 0u, 0u};
     if (kb == 4 && vb == 0) t.small_tile_cand = 3u, t.small_log2n = 22u;
-    if (kb == 4 && vb == 4) t.small_tile_cand = 3u, t.small_log2n = 25u;
+    if (kb == 4 && vb == 4) t.small_tile_cand = 1u, t.small_log2n = 25u, t.tiny_tile_cand = 3u, t.tiny_log2n = 22u;
     if (kb == 8 && vb == 8) t.small_tile_cand = 3u, t.small_log2n = 21u;
     return t;
 }
@@ -122,7 +125,7 @@ TuneEntry tune_default(uint32_t kb, uint32_t vb) {
 bool tune_valid(uint32_t kb, uint32_t vb, const TuneEntry& t) {
     return t.tile_cand < (uint32_t)kTileCands && t.hist_bpc >= 1 && t.hist_bpc <= 8 &&
            t.lds_threshold <= seg_cap((int)kb, (int)vb, kSegBins - 1) && t.small_tile_cand < (uint32_t)kTileCands &&
-           t.small_log2n <= 40u;
+           t.small_log2n <= 40u && t.tiny_tile_cand < (uint32_t)kTileCands && t.tiny_log2n <= 40u;
 }
 
 // Reads a table file into g_tune (caller holds g_tune_mu); returns entries read or -1.
@@ -134,11 +137,11 @@ int tune_read_file(const char* path) {
     while (fgets(line, sizeof(line), f)) {
         char* h = strchr(line, '#');
         if (h) *h = 0;
-        unsigned kb, vb, tc, bpc, thr, stc = 0, sl = 0;
-        const int got = sscanf(line, "%u %u %u %u %u %u %u", &kb, &vb, &tc, &bpc, &thr, &stc, &sl);
-        if (got != 5 && got != 7) continue;
+        unsigned kb, vb, tc, bpc, thr, stc = 0, sl = 0, ttc = 0, tl = 0;
+        const int got = sscanf(line, "%u %u %u %u %u %u %u %u %u", &kb, &vb, &tc, &bpc, &thr, &stc, &sl, &ttc, &tl);
+        if (got != 5 && got != 7 && got != 9) continue;
         const int ki = kb_index(kb), vi = vb_index(vb);
-        const TuneEntry t{tc, bpc, thr, stc, sl};
+        const TuneEntry t{tc, bpc, thr, stc, sl, got == 9 ? ttc : 0u, got == 9 ? tl : 0u};
         if (ki < 0 || vi < 0 || !tune_valid(kb, vb, t)) continue;
         g_tune[ki][vi] = t;
         ++n;
@@ -227,7 +230,9 @@ rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
     p->passes = p->kb;
     p->pass_kernel = pass_kernel_choice();
     const TuneEntry te = tune_get(p->kb, p->vb_core);
-    const uint32_t cand = (te.small_log2n && n < (1ull << te.small_log2n)) ? te.small_tile_cand : te.tile_cand;
+    const uint32_t cand = (te.tiny_log2n && n < (1ull << te.tiny_log2n))     ? te.tiny_tile_cand
+                          : (te.small_log2n && n < (1ull << te.small_log2n)) ? te.small_tile_cand
+                                                                             : te.tile_cand;
     p->tile_cand = tile_cands_built((int)p->kb, (int)p->vb_core) ? cand : 0u;
     p->hist_bpc = te.hist_bpc;
     if (p->pass_kernel == kPassChunk) {
@@ -733,6 +738,8 @@ rsort_status rsort_tuning_get(uint32_t key_bytes, uint32_t value_bytes, rsort_tu
     out->lds_threshold = t.lds_threshold;
     out->small_tile_candidate = t.small_tile_cand;
     out->small_log2n = t.small_log2n;
+    out->tiny_tile_candidate = t.tiny_tile_cand;
+    out->tiny_log2n = t.tiny_log2n;
     out->reserved = 0;
     return RSORT_SUCCESS;
 }
@@ -743,10 +750,10 @@ rsort_status rsort_tuning_set(uint32_t key_bytes, uint32_t value_bytes, const rs
     if (ki < 0 || vi < 0)
         return rsa::fail(RSORT_ERROR_UNSUPPORTED, "no tuning entry for key %u / value %u bytes", key_bytes, value_bytes);
     const rsa::TuneEntry e{t->tile_candidate, t->hist_blocks_per_cu, t->lds_threshold, t->small_tile_candidate,
-                           t->small_log2n};
+                           t->small_log2n,    t->tiny_tile_candidate, t->tiny_log2n};
     if (!rsa::tune_valid(key_bytes, value_bytes, e))
         return rsa::fail(RSORT_ERROR_INVALID_ARGUMENT, "tuning out of range (tile candidates < %d, 1..8 histogram "
-                         "workgroups per CU, LDS threshold <= %u, small_log2n <= 40)", rsa::kTileCands,
+                         "workgroups per CU, LDS threshold <= %u, small_log2n and tiny_log2n <= 40)", rsa::kTileCands,
                          rsa::seg_cap((int)key_bytes, (int)value_bytes, rsa::kSegBins - 1));
     std::lock_guard<std::mutex> lk(rsa::g_tune_mu);
     rsa::tune_ensure_locked();

@@ -105,7 +105,8 @@ class ServeStats(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("tile_candidate", ctypes.c_uint32), ("hist_blocks_per_cu", ctypes.c_uint32),
                 ("lds_threshold", ctypes.c_uint32), ("small_tile_candidate", ctypes.c_uint32),
-                ("small_log2n", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("small_log2n", ctypes.c_uint32), ("tiny_tile_candidate", ctypes.c_uint32),
+                ("tiny_log2n", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 SERVE_NO_GRAPH = 1
@@ -422,9 +423,11 @@ def tuning_get(key_bytes: int, value_bytes: int) -> Tuning:
 
 
 def tuning_set(key_bytes: int, value_bytes: int, tile_candidate: int, hist_blocks_per_cu: int,
-               lds_threshold: int, small_tile_candidate: int = 0, small_log2n: int = 0) -> None:
-    """Sets one entry; small_log2n = 0 uses tile_candidate for every n."""
-    t = Tuning(tile_candidate, hist_blocks_per_cu, lds_threshold, small_tile_candidate, small_log2n, 0)
+               lds_threshold: int, small_tile_candidate: int = 0, small_log2n: int = 0,
+               tiny_tile_candidate: int = 0, tiny_log2n: int = 0) -> None:
+    """Sets one entry; small_log2n = tiny_log2n = 0 uses tile_candidate for every n."""
+    t = Tuning(tile_candidate, hist_blocks_per_cu, lds_threshold, small_tile_candidate, small_log2n,
+               tiny_tile_candidate, tiny_log2n, 0)
     _check(lib().rsort_tuning_set(key_bytes, value_bytes, ctypes.byref(t)))
 
 

@@ -207,20 +207,25 @@ rsort_status rsort_digit_counts(const rsort_desc* desc, const void* keys, uint64
  *   small_tile_candidate, small_log2n
  *                       the tile candidate of sorts below 2^small_log2n keys
  *                       (few tiles: a smaller tile fills more CUs)
+ *   tiny_tile_candidate, tiny_log2n
+ *                       the same for sorts below 2^tiny_log2n keys, checked
+ *                       first (a second small-size tier)
  * Defaults are built in; at first use the library reads
  * $RSORT_AMD_TUNING_FILE, else tuning_gfx950.txt next to the library (lines
  * "key_bytes value_bytes tile_candidate hist_blocks_per_cu lds_threshold
- * [small_tile_candidate small_log2n]").
+ * [small_tile_candidate small_log2n [tiny_tile_candidate tiny_log2n]]").
  * Changing an entry changes workspace sizes: query again after a change.
  * The reference fixes these per build in a table for sm_86
  * (include/rsort/detail/tuning.cuh:138-229); there is no reference tuner.
  */
 typedef struct rsort_tuning {
-    uint32_t tile_candidate;       /* sorts of n >= 2^small_log2n keys */
+    uint32_t tile_candidate;       /* sorts above both small-size tiers */
     uint32_t hist_blocks_per_cu;
     uint32_t lds_threshold;
     uint32_t small_tile_candidate; /* sorts of n < 2^small_log2n keys */
-    uint32_t small_log2n;          /* 0: tile_candidate for every n */
+    uint32_t small_log2n;          /* 0: no small-size tier */
+    uint32_t tiny_tile_candidate;  /* sorts of n < 2^tiny_log2n keys (checked first) */
+    uint32_t tiny_log2n;           /* 0: no tiny-size tier */
     uint32_t reserved;             /* must be 0 */
 } rsort_tuning;
 

@@ -187,11 +187,15 @@ def test_tuning_defaults_set_and_plan(rs, tmp_path):
     rs.tuning_load("")  # built-in defaults only
     t = rs.tuning_get(4, 4)
     assert (t.tile_candidate, t.lds_threshold) == (0, 16384) and 1 <= t.hist_blocks_per_cu <= 8
-    assert (t.small_tile_candidate, t.small_log2n) == (3, 25)
+    assert (t.small_tile_candidate, t.small_log2n, t.tiny_tile_candidate, t.tiny_log2n) == (1, 25, 3, 22)
     d = rs.make_desc(0, 4, 4)
-    # below 2^25 keys the small-sort candidate (1024 x 10) is used
+    # below 2^22 keys the tiny tier (1024 x 10), below 2^25 the small tier (1024 x 16)
+    tiny = rs.plan(d, (1 << 22) - 1)
+    assert (tiny.threads, tiny.items_per_thread) == (1024, 10)
+    small = rs.plan(d, 1 << 22)
+    assert (small.threads, small.items_per_thread) == (1024, 16)
     small = rs.plan(d, (1 << 25) - 1)
-    assert (small.threads, small.items_per_thread) == (1024, 10)
+    assert (small.threads, small.items_per_thread) == (1024, 16)
     base = rs.plan(d, 1 << 25)
     assert (base.threads, base.items_per_thread) == (1024, 20)
     base = rs.plan(d, 1 << 27)
@@ -216,9 +220,13 @@ def test_tuning_defaults_set_and_plan(rs, tmp_path):
             1024, 16)
         # a table file round trip
         f = tmp_path / "t.txt"
-        f.write_text("# kb vb tile hist lds [small_tile small_log2n]\n4 4 1 3 4096\n8 0 2 2 8192 1 24\nbad line\n"
-                     "4 4 9 1 1\n8 8 0 1 1 0\n")
-        assert rs.tuning_load(str(f)) == 2
+        f.write_text("# kb vb tile hist lds [small_tile small_log2n [tiny_tile tiny_log2n]]\n4 4 1 3 4096\n"
+                     "8 0 2 2 8192 1 24\nbad line\n4 4 9 1 1\n8 8 0 1 1 0\n4 0 0 2 8192 1 24 3 21\n")
+        assert rs.tuning_load(str(f)) == 3
+        t40 = rs.tuning_get(4, 0)
+        assert (t40.small_tile_candidate, t40.small_log2n, t40.tiny_tile_candidate, t40.tiny_log2n) == (1, 24, 3, 21)
+        d40 = rs.make_desc(0, 4, 0)
+        assert [rs.plan(d40, 1 << l).items_per_thread for l in (20, 21, 23, 24)] == [8, 12, 12, 16]
         t = rs.tuning_get(4, 4)
         assert (t.tile_candidate, t.hist_blocks_per_cu, t.lds_threshold, t.small_log2n) == (1, 3, 4096, 0)
         t8 = rs.tuning_get(8, 0)
@@ -239,14 +247,16 @@ def test_persisted_table_is_read_at_first_use(rs):
     assert rs.tuning_load(None) == len(rows) > 0
     for row in rows:
         kb, vb, tc, bpc, thr = (int(x) for x in row[:5])
-        stc, sl = (int(row[5]), int(row[6])) if len(row) == 7 else (0, 0)
+        stc, sl = (int(row[5]), int(row[6])) if len(row) >= 7 else (0, 0)
+        ttc, tl = (int(row[7]), int(row[8])) if len(row) == 9 else (0, 0)
         t = rs.tuning_get(kb, vb)
         assert (t.tile_candidate, t.hist_blocks_per_cu, t.lds_threshold) == (tc, bpc, thr)
-        assert (t.small_tile_candidate, t.small_log2n) == (stc, sl)
+        assert (t.small_tile_candidate, t.small_log2n, t.tiny_tile_candidate, t.tiny_log2n) == (stc, sl, ttc, tl)
 
 
 @pytest.mark.parametrize("args", [(4, 4, 4, 1, 0), (4, 4, 0, 0, 0), (4, 4, 0, 9, 0), (4, 4, 0, 1, 16385),
-                                  (3, 4, 0, 1, 0), (4, 12, 0, 1, 0), (4, 4, 0, 1, 0, 4, 20), (4, 4, 0, 1, 0, 1, 41)])
+                                  (3, 4, 0, 1, 0), (4, 12, 0, 1, 0), (4, 4, 0, 1, 0, 4, 20), (4, 4, 0, 1, 0, 1, 41),
+                                  (4, 4, 0, 1, 0, 1, 24, 4, 20), (4, 4, 0, 1, 0, 1, 24, 3, 41)])
 def test_tuning_rejects_out_of_range(rs, args):
     with pytest.raises(rs.RsortError):
         rs.tuning_set(*args)
@@ -287,16 +297,15 @@ def test_autotune_small_choice():
     at = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(at)
     # whole-sort microseconds by log2 n and candidate, u32 keys and u32/u32 on MI355X
-    keys = {20: {0: 85.7, 1: 76.3, 2: 79.9, 3: 72.6}, 21: {0: 98.7, 1: 93.3, 2: 97.7, 3: 91.2},
-            22: {0: 112.8, 1: 121.0, 2: 125.5, 3: 121.5}, 23: {0: 149.1, 1: 163.2, 2: 179.6, 3: 170.4},
-            24: {0: 229.8, 1: 239.2, 2: 246.8, 3: 255.2}}
-    assert at.small_choice(keys, 0) == (3, 22)
-    pairs = {20: {0: 109.8, 1: 100.9, 2: 99.8, 3: 87.0}, 22: {0: 137.0, 1: 119.9, 2: 133.7, 3: 132.6},
-             24: {0: 341.8, 1: 314.0, 2: 342.1, 3: 329.1}, 26: {0: 1190.0, 1: 1230.0, 2: 1260.0, 3: 1300.0}}
-    # cand 3 below 2^25 sums to 4.154 relative, cand 1 below 2^25 to 4.160
-    assert at.small_choice(pairs, 0) == (3, 25)
-    pairs[20][3] = 95.0
-    assert at.small_choice(pairs, 0) == (1, 25)
-    # the large-sort candidate fastest everywhere: no small candidate
+    keys = {20: {0: 82.4, 1: 77.5, 2: 74.7, 3: 72.1}, 21: {0: 99.0, 1: 95.9, 2: 96.1, 3: 87.8},
+            22: {0: 113.2, 1: 119.8, 2: 121.3, 3: 122.9}, 23: {0: 147.9, 1: 161.2, 2: 179.1, 3: 167.3},
+            24: {0: 223.3, 1: 233.6, 2: 244.6, 3: 249.3}, 26: {0: 687.5, 1: 743.1, 2: 781.7, 3: 853.8}}
+    assert at.small_choice(keys, 0) == (0, 0, 3, 22)  # one tier: candidate 3 below 2^22
+    pairs = {20: {0: 108.9, 1: 101.4, 2: 99.7, 3: 87.1}, 21: {0: 118.6, 1: 114.2, 2: 115.7, 3: 97.6},
+             22: {0: 136.9, 1: 121.8, 2: 132.7, 3: 133.6}, 23: {0: 201.8, 1: 173.8, 2: 206.5, 3: 188.7},
+             24: {0: 342.6, 1: 309.8, 2: 343.1, 3: 330.7}, 26: {0: 1196.7, 1: 1223.0, 2: 1393.3, 3: 1311.3}}
+    # two tiers: candidate 3 below 2^22, candidate 1 below 2^25
+    assert at.small_choice(pairs, 0) == (3, 22, 1, 25)
+    # the large-sort candidate fastest everywhere: no tier
     flat = {20: {0: 1.0, 1: 2.0}, 22: {0: 1.0, 1: 1.5}}
-    assert at.small_choice(flat, 0) == (0, 0)
+    assert at.small_choice(flat, 0) == (0, 0, 0, 0)
