@@ -1,0 +1,61 @@
+"""Parity of the persistent chunked pass kernel against the oracle.
+
+The library reads RSORT_AMD_PASS_KERNEL once per process, so
+test_gpu_chunk_kernel.py runs this file in a process of its own with
+RSORT_AMD_PASS_KERNEL=chunk.  Cases: u32 keys and u32/u32 pairs in both
+orders, u64/u64 pairs, signed keys, and keys with an odd count of
+non-uniform passes (the lowest uniform pass runs as a copy)."""
+from __future__ import annotations
+
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+for p in (HERE, ROOT, os.path.join(ROOT, "cuda-fast-sort_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+import torch  # noqa: E402
+
+import rsort_amd as rs  # noqa: E402
+from helpers import index_values, random_keys  # noqa: E402
+from oracle import oracle as orc  # noqa: E402
+
+
+def check(keys, vals, kind, kb, vb, desc):
+    n = keys.size // kb
+    print(f"[case] chunk kernel kind={kind} kb={kb} vb={vb} desc={int(desc)} n={n}", flush=True)
+    dk = torch.from_numpy(keys.copy()).cuda()
+    dv = torch.from_numpy(vals.copy()).cuda() if vals is not None else None
+    d = rs.make_desc(kind, kb, vb, desc)
+    tb = rs.sort_raw(d, None, None, None, 0, n)
+    ws = torch.empty(tb, dtype=torch.uint8, device="cuda")
+    rs.sort_raw(d, dk.data_ptr(), dv.data_ptr() if dv is not None else None, ws.data_ptr(), tb, n)
+    torch.cuda.synchronize()
+    ek, ev = orc.sort(keys, vals, kind, kb, desc, vb=vb if vb else None)
+    assert np.array_equal(dk.cpu().numpy(), ek), "keys differ"
+    if vb:
+        assert np.array_equal(dv.cpu().numpy(), ev), "values differ"
+
+
+def main():
+    assert os.environ.get("RSORT_AMD_PASS_KERNEL") == "chunk"
+    assert torch.cuda.is_available()
+    rs.lib()
+    orc.lib()
+    u8 = lambda a: np.ascontiguousarray(a).view(np.uint8).reshape(-1)
+    for n, desc in ((1000, False), (300_007, False), (300_007, True)):
+        check(u8(random_keys(0, 4, n, seed=n)), None, 0, 4, 0, desc)
+        check(u8(random_keys(0, 4, n, seed=n + 1)), u8(index_values(4, n)), 0, 4, 4, desc)
+    check(u8(random_keys(0, 8, 100_003, seed=5)), u8(index_values(8, 100_003)), 0, 8, 8, False)
+    check(u8(random_keys(1, 4, 200_001, seed=6)), u8(index_values(4, 200_001)), 1, 4, 4, True)
+    r = orc.mt19937(3, 200_000)
+    k_odd = ((r & 0xFF000000) | 0x00123456).astype(np.uint32)
+    check(u8(k_odd), u8(index_values(4, 200_000)), 0, 4, 4, False)
+    print("chunk kernel parity ok", flush=True)
+
+
+if __name__ == "__main__":
+    main()
