@@ -503,29 +503,30 @@ __device__ __forceinline__ uint32_t reg_xor(uint32_t a, uint32_t b) { return a ^
 __device__ __forceinline__ uint64_t reg_xor(uint64_t a, uint64_t b) { return a ^ b; }
 __device__ __forceinline__ U128 reg_xor(const U128& a, const U128& b) { return U128{a.lo ^ b.lo, a.hi ^ b.hi}; }
 
-// Count pass P of V keys held by each lane.  If the whole wave's group has one
-// digit in this pass, a single lane adds the group size (avoids 64-way LDS
-// atomic serialisation on uniform bytes).
-template <int KB, int P, int V>
+// Count pass P of V keys held by each lane; counter (P, digit) is word
+// (P * 256 + digit) * STRIDE from H.  If the whole wave's group has one digit
+// in this pass, a single lane adds the group size (avoids 64-way LDS atomic
+// serialisation on uniform bytes).
+template <int KB, int P, int V, int STRIDE>
 __device__ __forceinline__ void hist_count_pass(uint32_t* H, const typename KeyTraits<KB>::Reg (&k)[V],
                                                 const typename KeyTraits<KB>::Reg& v,
                                                 const typename KeyTraits<KB>::Reg& o0) {
     if constexpr (P < KB) {
         if (!__any(reg_byte<P>(v) != 0u)) {
-            if (lane_id() == 0) atomicAdd(H + P * 256 + reg_byte<P>(o0), (uint32_t)(64 * V));
+            if (lane_id() == 0) atomicAdd(H + (P * 256 + reg_byte<P>(o0)) * STRIDE, (uint32_t)(64 * V));
         } else {
 #pragma unroll
-            for (int j = 0; j < V; ++j) atomicAdd(H + P * 256 + reg_byte<P>(k[j]), 1u);
+            for (int j = 0; j < V; ++j) atomicAdd(H + (P * 256 + reg_byte<P>(k[j])) * STRIDE, 1u);
         }
-        hist_count_pass<KB, P + 1, V>(H, k, v, o0);
+        hist_count_pass<KB, P + 1, V, STRIDE>(H, k, v, o0);
     }
 }
 
-template <int KB, int P>
+template <int KB, int P, int STRIDE>
 __device__ __forceinline__ void hist_count_one(uint32_t* H, const typename KeyTraits<KB>::Reg& k) {
     if constexpr (P < KB) {
-        atomicAdd(H + P * 256 + reg_byte<P>(k), 1u);
-        hist_count_one<KB, P + 1>(H, k);
+        atomicAdd(H + (P * 256 + reg_byte<P>(k)) * STRIDE, 1u);
+        hist_count_one<KB, P + 1, STRIDE>(H, k);
     }
 }
 
@@ -536,13 +537,26 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
     constexpr int P = KB;
     constexpr int WAVES = kHistThreads / 64;
-    constexpr int COPIES = hist_copies(KB);
-    constexpr int WPC = WAVES / COPIES;
     constexpr int V = HistVec<KB>::V;
     constexpr uint32_t G = 64u * V;  // keys per wave group
     constexpr int UNROLL = 4;  // 16-byte loads per lane per batch
+#if RSA_HIST_SLOTS
+    // Lane slots: counter (pass, digit) has SLOTS copies side by side and lane
+    // l adds to copy l mod SLOTS, so the 32 lanes an LDS atomic serves at once
+    // hit distinct banks whatever their digits (32 slots), or at most
+    // 32 / SLOTS of them share one.
+    constexpr int SLOTS = hist_slots(KB);
+    constexpr int STRIDE = SLOTS;
+    constexpr uint32_t HWORDS = (uint32_t)P * 256u * SLOTS;
+#else
+    // One copy of the counters per wave (pairs of waves for wide keys).
+    constexpr int COPIES = hist_copies(KB);
+    constexpr int WPC = WAVES / COPIES;
+    constexpr int STRIDE = 1;
+    constexpr uint32_t HWORDS = (uint32_t)COPIES * P * 256u;
+#endif
 
-    __shared__ uint32_t s_hist[COPIES * P * 256];
+    __shared__ uint32_t s_hist[HWORDS];
     __shared__ Off s_wsum[4];
     __shared__ uint32_t s_last;
     __shared__ uint32_t s_skip;
@@ -552,11 +566,27 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     const uint32_t lane = tid & 63u;
     RSA_TIMER_START(a.dbg);
 
-    for (uint32_t i = tid; i < (uint32_t)(COPIES * P * 256); i += kHistThreads) s_hist[i] = 0u;
+    for (uint32_t i = tid; i < HWORDS; i += kHistThreads) s_hist[i] = 0u;
     if (tid == 0) s_skip = 0u;
     __syncthreads();
 
+#if RSA_HIST_SLOTS
+    uint32_t* H = s_hist + (lane & (SLOTS - 1));
+#else
     uint32_t* H = s_hist + (w / WPC) * (P * 256);
+#endif
+    // this workgroup's total of counter i = pass * 256 + digit
+    auto wg_total = [&](uint32_t i) {
+        uint32_t s = 0;
+#if RSA_HIST_SLOTS
+#pragma unroll
+        for (int c = 0; c < SLOTS; ++c) s += s_hist[i * SLOTS + ((c + i) & (SLOTS - 1))];  // rotated: no conflicts
+#else
+#pragma unroll
+        for (int c = 0; c < COPIES; ++c) s += s_hist[c * P * 256 + i];
+#endif
+        return s;
+    };
     const Mem* keys = (const Mem*)a.keys;
     const bool twid = a.tw.active != 0u;
 
@@ -582,7 +612,7 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
         }
         const Reg o0 = rfl(o);
         const Reg v = reg_or(reg_xor(o, an), reg_xor(o, o0));
-        hist_count_pass<KB, 0, V>(H, k, v, o0);
+        hist_count_pass<KB, 0, V, STRIDE>(H, k, v, o0);
     };
     constexpr uint64_t STEP = (uint64_t)UNROLL * WAVES;
     uint64_t g = g0 + w;
@@ -623,7 +653,7 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
             const uint64_t idx = t < a.head ? t : tail0 + (t - a.head);
             Reg k = key_to_reg<KB>(keys[idx]);
             if (twid) k = tw_in<KB>(k, a.tw);
-            hist_count_one<KB, 0>(H, k);
+            hist_count_one<KB, 0, STRIDE>(H, k);
         }
     }
     __syncthreads();
@@ -637,9 +667,7 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
         // with device atomics, and the last arrival finalises (three coherent
         // round trips fewer than the two-level fold below).
         for (uint32_t i = tid; i < (uint32_t)(P * 256); i += kHistThreads) {
-            uint32_t s = 0;
-#pragma unroll
-            for (int c = 0; c < COPIES; ++c) s += s_hist[c * P * 256 + i];
+            const uint32_t s = wg_total(i);
             if (s) atomicAdd(counts + i, (Off)s);
         }
         __builtin_amdgcn_s_waitcnt(0);  // device atomics on the counts have completed
@@ -662,12 +690,8 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
         // to finish finalises.
         {
             uint32_t* row = a.partials + (size_t)blockIdx.x * (P * 256);
-            for (uint32_t i = tid; i < (uint32_t)(P * 256); i += kHistThreads) {
-                uint32_t s = 0;
-#pragma unroll
-                for (int c = 0; c < COPIES; ++c) s += s_hist[c * P * 256 + i];
-                __hip_atomic_store(row + i, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
+            for (uint32_t i = tid; i < (uint32_t)(P * 256); i += kHistThreads)
+                __hip_atomic_store(row + i, wg_total(i), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         const uint32_t group = blockIdx.x / kHistGroup;
         const uint32_t grp0 = group * kHistGroup;

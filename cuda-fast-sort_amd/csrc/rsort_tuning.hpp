@@ -12,7 +12,13 @@
 
 namespace rsa {
 
-constexpr int kHistThreads = 512;
+// Histogram counter layout: 1 = lane slots (bank-conflict-free LDS atomics,
+// one 1024-thread workgroup per CU), 0 = one counter copy per wave (512
+// threads, several workgroups per CU).
+#ifndef RSA_HIST_SLOTS
+#define RSA_HIST_SLOTS 1
+#endif
+constexpr int kHistThreads = RSA_HIST_SLOTS ? 1024 : 512;
 // Histogram workgroups fold their counts in groups of this many (the last
 // arrival of a group sums the group's rows) before touching global atomics.
 constexpr int kHistGroup = 32;
@@ -41,9 +47,14 @@ RSA_HD constexpr int hist_copies(int key_bytes) {
                                                                    : (64 * 1024) / (key_bytes * 1024);
 }
 
+// Lane slots per histogram counter: 32 (one per LDS bank) while the
+// counters fit in 128 KiB, fewer for 8- and 16-byte keys.
+RSA_HD constexpr int hist_slots(int key_bytes) { return key_bytes <= 4 ? 32 : 128 / key_bytes; }
+
 // Histogram workgroups per CU: as many as the LDS counters allow, up to 4.
 RSA_HD constexpr uint32_t hist_blocks_per_cu(int key_bytes) {
-    const uint32_t lds = (uint32_t)hist_copies(key_bytes) * (uint32_t)key_bytes * 1024u;
+    const uint32_t lds = RSA_HIST_SLOTS ? (uint32_t)hist_slots(key_bytes) * (uint32_t)key_bytes * 1024u
+                                        : (uint32_t)hist_copies(key_bytes) * (uint32_t)key_bytes * 1024u;
     const uint32_t b = (160u * 1024u - 1024u) / lds;
     return b < 1u ? 1u : (b > 4u ? 4u : b);
 }
