@@ -259,3 +259,15 @@ def test_histogram_blocks_per_cu(rs, orc, bpc):
         check(rs, orc, keys, index_values(4, 1_000_003), 0, 4, 4, False)
     finally:
         rs.tuning_load(None)
+
+
+@pytest.mark.parametrize("n", [1 << 19, (1 << 19) + 1, 1 << 22, (1 << 22) + 1])
+def test_histogram_fold_boundaries(rs, orc, n):
+    """u32 keys at the histogram grid sizes where the fold changes (16384 keys
+    per 1024-thread workgroup, two per CU): 32 workgroups fold as one group,
+    33 to 256 add to the global counts directly, 257 and more fold in two
+    levels."""
+    keys = random_keys(0, 4, n, seed=n % 1000003)
+    check(rs, orc, keys, None, 0, 4, 0, n % 2 == 1)
+    if n < (1 << 22):
+        check(rs, orc, keys, index_values(4, n), 0, 4, 4, False)
