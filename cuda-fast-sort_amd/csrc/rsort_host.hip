@@ -258,9 +258,10 @@ rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
     // fill kernel clears them at the start of every sort
     p->off_ctrl = reserve(&off, sizeof(Ctrl));
     p->off_counts = reserve(&off, (size_t)p->passes * kRadix * off_sz, 16);
-    // lookback: tile words, then per-pass group words (kLbGroup tiles each)
+    // lookback: the per-pass tile ticket counters, tile words, then per-pass
+    // group words (kLbGroup tiles each)
     const size_t groups = (size_t)((p->num_tiles + kLbGroup - 1) / kLbGroup);
-    p->lb_group_off = (size_t)p->num_tiles * kRadix * off_sz;
+    p->lb_group_off = kPartTicketBytes + (size_t)p->num_tiles * kRadix * off_sz;
     p->lb_bytes = p->lb_group_off + (size_t)p->passes * groups * kRadix * off_sz;
     p->off_lb = reserve(&off, p->lb_bytes, 16);
     if (p->off_counts != p->off_ctrl + sizeof(Ctrl) || p->off_lb != p->off_counts + (size_t)p->passes * kRadix * off_sz)
@@ -432,7 +433,8 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
     j.counts = base + p.off_counts;
     j.gp = base + p.off_gp;
     j.hist_partials = (uint32_t*)(base + p.off_hist_partials);
-    j.lookback = base + p.off_lb;
+    j.part_tickets = (uint32_t*)(base + p.off_lb);
+    j.lookback = base + p.off_lb + kPartTicketBytes;
     j.lb_bytes = p.lb_bytes;
     j.lb_groups = base + p.off_lb + p.lb_group_off;
     j.n = n;

@@ -347,6 +347,7 @@ struct PassParams {
     const void* gp;        // [passes][256] exclusive digit offsets
     void* lookback;        // [tiles][256] words
     void* lb_groups;       // [passes][groups][256] group words (kLbGroup tiles each), zeroed per sort
+    uint32_t* part_tickets;  // [passes][kTicketParts] tile ticket counters, zeroed per sort
     Ctrl* ctrl;
     uint64_t n;
     uint32_t num_tiles;
@@ -966,7 +967,21 @@ __global__ __launch_bounds__(THREADS, pass_min_waves(KB, VB, K)) void k_onesweep
 #ifdef RSA_PHASE_TIMING
     if (threadIdx.x < 2) L.dbg[threadIdx.x] = 0u;
 #endif
-    const uint32_t tile = onesweep_begin(L, &a.ctrl->ticket[a.pass]);
+    constexpr uint32_t C = tile_chunk(KB);
+    uint32_t tile;
+    if constexpr (C > 0) {
+        // ticket k of counter `part` is tile (k / C) * (parts * C) + part * C + k % C;
+        // each counter hands out its tiles in increasing order, so a tile's
+        // predecessors have all been handed out before it.  The grid is rounded
+        // up to whole rounds of parts * C tiles: workgroups past the last tile
+        // end here.
+        const uint32_t part = blockIdx.x % kTicketParts;
+        const uint32_t tk = onesweep_begin(L, a.part_tickets + a.pass * kTicketParts + part);
+        tile = (tk / C) * (kTicketParts * C) + part * C + tk % C;
+        if (tile >= a.num_tiles) return;
+    } else {
+        tile = onesweep_begin(L, &a.ctrl->ticket[a.pass]);
+    }
     TileGeom g;
     g.lb_tile = tile;
     g.head = tile == 0u;

@@ -33,8 +33,9 @@ struct SortJob {
     void* gp;
     uint32_t* hist_partials;    // [hist grid][passes][256]
     void* lookback;
-    uint64_t lb_bytes;    // tile words and group words (cleared together)
+    uint64_t lb_bytes;    // ticket counters, tile words and group words (cleared together)
     void* lb_groups;      // group words: [passes][groups][256]
+    uint32_t* part_tickets;  // per-pass, per-XCD tile ticket counters (start of the lookback region)
     uint64_t n;
     uint64_t num_tiles;
     uint32_t passes;

@@ -11,15 +11,17 @@ namespace rsa {
 // segmented lookback walk.
 template <int KB, int VB, bool WIDE, int THREADS, int K>
 void launch_onesweep(const SortJob& j, const PassParams& pp, hipStream_t s) {
+    // whole rounds of the per-XCD ticket counters (extra workgroups end at once)
+    constexpr uint32_t C = tile_chunk(KB);
+    const uint64_t round = C ? (uint64_t)kTicketParts * C : 1u;
+    const uint32_t grid = (uint32_t)((j.num_tiles + round - 1) / round * round);
     if constexpr (!WIDE && THREADS >= 512) {
         if (j.lb_wide) {
-            hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, true>), dim3((uint32_t)j.num_tiles),
-                               dim3(THREADS), 0, s, pp);
+            hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, true>), dim3(grid), dim3(THREADS), 0, s, pp);
             return;
         }
     }
-    hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, false>), dim3((uint32_t)j.num_tiles), dim3(THREADS), 0,
-                       s, pp);
+    hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, false>), dim3(grid), dim3(THREADS), 0, s, pp);
 }
 
 template <int KB, int VB, bool WIDE>
@@ -72,6 +74,7 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     pp.gp = j.gp;
     pp.lookback = j.lookback;
     pp.lb_groups = j.lb_groups;
+    pp.part_tickets = j.part_tickets;
     pp.ctrl = j.ctrl;
     pp.n = j.n;
     pp.num_tiles = (uint32_t)j.num_tiles;

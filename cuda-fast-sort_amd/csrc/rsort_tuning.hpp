@@ -30,6 +30,24 @@ constexpr uint32_t kHistMaxGrid = 32 * kHistGroup;
 // the device atomics few).
 constexpr uint32_t kHistDirectMaxGrid = 256;
 
+// Tile tickets of the pass kernel, by XCD: workgroup b takes its tickets
+// from counter b mod kTicketParts (the round-robin order in which workgroups
+// are placed on the 8 XCDs), and each counter hands out runs of
+// tile_chunk() consecutive tiles, so that neighbouring tiles -- whose digit
+// runs share the partial cache lines at their ends -- are written through
+// the same L2.  Measured on MI355X: runs of 8 tiles for 1- to 4-byte keys,
+// 16 for 8- and 16-byte keys.  RSA_XCD_CHUNK > 0 fixes the run length;
+// RSA_XCD_CHUNK 0 = one global ticket counter.
+#ifndef RSA_XCD_CHUNK
+#define RSA_XCD_CHUNK -1
+#endif
+constexpr uint32_t kTicketParts = 8;
+RSA_HD constexpr uint32_t tile_chunk(int key_bytes) {
+    return RSA_XCD_CHUNK >= 0 ? (uint32_t)RSA_XCD_CHUNK : (key_bytes >= 8 ? 16u : 8u);
+}
+// bytes of the per-pass ticket counters at the start of the lookback region
+constexpr size_t kPartTicketBytes = 16 * kTicketParts * 4;  // 16 passes at most
+
 // Passes of at most this many tiles use the one-round segmented lookback
 // walk (all of their tiles are resident at once on 256 CUs).
 constexpr uint64_t kLbWideMaxTiles = 512;

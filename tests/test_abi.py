@@ -57,10 +57,11 @@ def test_plan_switches_to_wide_words(rs):
     assert rs.plan(d, (1 << 29) - 1).wide == 0
     assert rs.plan(d, 1 << 29).wide == 1
     small, big = rs.plan(d, (1 << 29) - 1), rs.plan(d, 1 << 29)
-    # tile words plus one group word per 8 tiles for every pass, 8 bytes each
+    # 512 bytes of tile ticket counters (8 per pass, 16 passes at most), tile
+    # words, and one group word per 8 tiles for every pass, 8 bytes each
     groups = (big.num_tiles + 7) // 8
-    assert big.lookback_bytes == (big.num_tiles + big.passes * groups) * 256 * 8
-    assert small.lookback_bytes == (small.num_tiles + small.passes * ((small.num_tiles + 7) // 8)) * 256 * 4
+    assert big.lookback_bytes == 512 + (big.num_tiles + big.passes * groups) * 256 * 8
+    assert small.lookback_bytes == 512 + (small.num_tiles + small.passes * ((small.num_tiles + 7) // 8)) * 256 * 4
 
 
 @pytest.mark.parametrize("field,value", [("key_kind", 3), ("key_bytes", 3), ("key_bytes", 32), ("descending", 2),
