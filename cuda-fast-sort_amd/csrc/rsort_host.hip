@@ -111,13 +111,13 @@ TuneEntry g_tune[5][6];
 bool g_tune_ready = false;
 
 // Measured on MI355X (whole-call time, uniform keys): u32 keys sort faster
-// with 1024 x 8 tiles below 2^22 keys; u32/u32 with 1024 x 10 tiles below
-// 2^22 and 1024 x 16 tiles (two workgroups per CU) below 2^25; u64/u64 with
+// with 1024 x 8 tiles below 2^22 keys; u32/u32 with 1024 x 16 tiles (two
+// workgroups per CU) from 2^22 up and 1024 x 10 tiles below; u64/u64 with
 // 1024 x 5 tiles below 2^21.
 TuneEntry tune_default(uint32_t kb, uint32_t vb) {
     TuneEntry t{0u, hist_blocks_per_cu((int)kb), seg_cap((int)kb, (int)vb, kSegBins - 1), 0u, 0u, 0u, 0u};
     if (kb == 4 && vb == 0) t.small_tile_cand = 3u, t.small_log2n = 22u;
-    if (kb == 4 && vb == 4) t.small_tile_cand = 1u, t.small_log2n = 25u, t.tiny_tile_cand = 3u, t.tiny_log2n = 22u;
+    if (kb == 4 && vb == 4) t.tile_cand = 1u, t.small_tile_cand = 3u, t.small_log2n = 22u;
     if (kb == 8 && vb == 8) t.small_tile_cand = 3u, t.small_log2n = 21u;
     return t;
 }

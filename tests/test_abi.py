@@ -187,18 +187,18 @@ def test_serve_rejects_bad_config_before_device_work(rs):
 def test_tuning_defaults_set_and_plan(rs, tmp_path):
     rs.tuning_load("")  # built-in defaults only
     t = rs.tuning_get(4, 4)
-    assert (t.tile_candidate, t.lds_threshold) == (0, 16384) and 1 <= t.hist_blocks_per_cu <= 8
-    assert (t.small_tile_candidate, t.small_log2n, t.tiny_tile_candidate, t.tiny_log2n) == (1, 25, 3, 22)
+    assert (t.tile_candidate, t.lds_threshold) == (1, 16384) and 1 <= t.hist_blocks_per_cu <= 8
+    assert (t.small_tile_candidate, t.small_log2n, t.tiny_tile_candidate, t.tiny_log2n) == (3, 22, 0, 0)
     d = rs.make_desc(0, 4, 4)
-    # below 2^22 keys the tiny tier (1024 x 10), below 2^25 the small tier (1024 x 16)
-    tiny = rs.plan(d, (1 << 22) - 1)
-    assert (tiny.threads, tiny.items_per_thread) == (1024, 10)
-    small = rs.plan(d, 1 << 22)
-    assert (small.threads, small.items_per_thread) == (1024, 16)
-    small = rs.plan(d, (1 << 25) - 1)
-    assert (small.threads, small.items_per_thread) == (1024, 16)
-    base = rs.plan(d, 1 << 25)
-    assert (base.threads, base.items_per_thread) == (1024, 20)
+    # below 2^22 keys the small tier (1024 x 10), from there candidate 1 (1024 x 16)
+    small = rs.plan(d, (1 << 22) - 1)
+    assert (small.threads, small.items_per_thread) == (1024, 10)
+    base = rs.plan(d, 1 << 22)
+    assert (base.threads, base.items_per_thread) == (1024, 16)
+    base = rs.plan(d, 1 << 28)
+    assert (base.threads, base.items_per_thread) == (1024, 16)
+    # candidate 0, the built-in geometry
+    rs.tuning_set(4, 4, 0, t.hist_blocks_per_cu, 16384)
     base = rs.plan(d, 1 << 27)
     assert (base.threads, base.items_per_thread) == (1024, 20)
     try:
@@ -236,7 +236,7 @@ def test_tuning_defaults_set_and_plan(rs, tmp_path):
         assert rs.tuning_load(str(tmp_path / "missing.txt")) == -1
     finally:
         rs.tuning_load(None)
-    assert rs.plan(d, 1 << 20).items_per_thread == 10 and rs.plan(d, 1 << 28).items_per_thread == 20
+    assert rs.plan(d, 1 << 20).items_per_thread == 10 and rs.plan(d, 1 << 28).items_per_thread == 16
 
 
 def test_persisted_table_is_read_at_first_use(rs):
