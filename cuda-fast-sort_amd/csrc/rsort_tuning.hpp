@@ -35,16 +35,15 @@ constexpr uint32_t kHistDirectMaxGrid = 256;
 // are placed on the 8 XCDs), and each counter hands out runs of
 // tile_chunk() consecutive tiles, so that neighbouring tiles -- whose digit
 // runs share the partial cache lines at their ends -- are written through
-// the same L2.  Measured on MI355X: runs of 8 tiles for 1- to 4-byte keys,
-// 16 for 8- and 16-byte keys.  RSA_XCD_CHUNK > 0 fixes the run length;
-// RSA_XCD_CHUNK 0 = one global ticket counter.
+// the same L2.  Measured on MI355X with the tuned tiles: runs of 16 tiles;
+// runs of 8 were 1-2 % slower for u32/u32 and 3 % for u64, runs of 32 and 2
+// slower still.  RSA_XCD_CHUNK sets the run length; 0 = one global ticket
+// counter.
 #ifndef RSA_XCD_CHUNK
-#define RSA_XCD_CHUNK -1
+#define RSA_XCD_CHUNK 16
 #endif
 constexpr uint32_t kTicketParts = 8;
-RSA_HD constexpr uint32_t tile_chunk(int key_bytes) {
-    return RSA_XCD_CHUNK >= 0 ? (uint32_t)RSA_XCD_CHUNK : (key_bytes >= 8 ? 16u : 8u);
-}
+RSA_HD constexpr uint32_t tile_chunk(int) { return (uint32_t)RSA_XCD_CHUNK; }
 // bytes of the per-pass ticket counters at the start of the lookback region
 constexpr size_t kPartTicketBytes = 16 * kTicketParts * 4;  // 16 passes at most
 
