@@ -71,6 +71,16 @@ uint64_t lb_wide_max_tiles() {
     return v;
 }
 
+// u32/u32 passes of at least this many tiles publish tile aggregates before
+// the rank.  RSORT_AMD_EARLY_AGG_TILES overrides it (0 = never).
+uint64_t early_agg_min_tiles() {
+    static const uint64_t v = [] {
+        const char* e = getenv("RSORT_AMD_EARLY_AGG_TILES");
+        return e ? (uint64_t)strtoull(e, nullptr, 10) : (uint64_t)kEarlyAggMinTiles;
+    }();
+    return v;
+}
+
 // Histogram grids of more than one fold group and at most this many
 // workgroups add their counts with device atomics directly instead of the
 // two-level fold.  RSORT_AMD_HIST_DIRECT overrides it (0 = never).
@@ -423,6 +433,8 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
     j.wide = p.wide;
     j.allow_skip = p.allow_skip;
     j.lb_wide = p.lb_wide;
+    j.early_agg = p.kb == 4 && p.vb_core == 4 && !p.wide && early_agg_min_tiles() > 0 &&
+                  p.num_tiles >= early_agg_min_tiles();
     j.hist_direct_max = hist_direct_max();
     j.tw = make_tw_masks(p.kind, p.kb, p.desc);
     j.keys = keys;

@@ -843,6 +843,7 @@ struct OnesweepLds {
     Off base[256];
     uint32_t wsum[4];
     uint32_t tile;
+    uint32_t early[256];  // tile digit counts taken before the rank (early aggregate)
 #ifdef RSA_PHASE_TIMING
     uint32_t dbg[2];
 #endif
@@ -886,6 +887,7 @@ __device__ __forceinline__ uint32_t onesweep_begin(OnesweepLds<KB, VB, WIDE, THR
         constexpr uint32_t cnt_units = 256u * sizeof(L.cnt[0]) / 16u;
         if (lane < cnt_units) ((uint4*)(L.cnt + w * 256))[lane] = make_uint4(0u, 0u, 0u, 0u);
     }
+    if (tid < 256) L.early[tid] = 0u;
     __syncthreads();
     return L.tile;
 }
@@ -932,8 +934,9 @@ __device__ __forceinline__ void copy_pass_range(const PassIo<KB, VB>& io, const 
 
 // One LSD pass over the whole array: one workgroup per tile, tiles taken in
 // ticket order so every tile's predecessors are already running.  SEGWALK:
-// the lookback walk in one-round segments (passes of few tiles).
-template <int KB, int VB, bool WIDE, int THREADS, int K, bool SEGWALK>
+// the lookback walk in one-round segments (passes of few tiles).  EARLY: the
+// tile aggregate is published from digit counts taken before the rank.
+template <int KB, int VB, bool WIDE, int THREADS, int K, bool SEGWALK, bool EARLY>
 __global__ __launch_bounds__(THREADS, pass_min_waves(KB, VB, K)) void k_onesweep(PassParams a) {
     using KM = typename KeyTraits<KB>::Mem;
     using KR = typename KeyTraits<KB>::Reg;
@@ -992,6 +995,7 @@ __global__ __launch_bounds__(THREADS, pass_min_waves(KB, VB, K)) void k_onesweep
     constexpr bool LB_GROUPS = true;
     Off* lbg = (Off*)a.lb_groups + (size_t)a.pass * ((a.num_tiles + kLbGroup - 1) / kLbGroup) * 256u;
     constexpr bool LB_SEGWALK = SEGWALK;
+    constexpr bool EARLY_AGG = EARLY;
     const TwMasks& tw = a.tw;
     const uint32_t tile_tid = threadIdx.x;
 #include "rsort_onesweep_tile.inc"

@@ -22,6 +22,7 @@ struct SortJob {
     bool wide;            // 64-bit offsets and lookback words
     bool allow_skip;      // skip passes whose digit is uniform
     bool lb_wide;         // one-round segmented lookback walk (passes of few tiles)
+    bool early_agg;       // tile aggregates from digit counts taken before the rank
     uint32_t hist_direct_max;  // histogram grids up to this fold by direct device atomics
     TwMasks tw;
     void* keys;           // caller buffer (in/out)

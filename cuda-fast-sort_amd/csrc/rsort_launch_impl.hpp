@@ -8,7 +8,8 @@
 namespace rsa {
 
 // One pass of the tile kernel; passes of few tiles take the variant with the
-// segmented lookback walk.
+// segmented lookback walk, large u32/u32 passes the one that publishes tile
+// aggregates before the rank.
 template <int KB, int VB, bool WIDE, int THREADS, int K>
 void launch_onesweep(const SortJob& j, const PassParams& pp, hipStream_t s) {
     // whole rounds of the per-XCD ticket counters (extra workgroups end at once)
@@ -17,11 +18,19 @@ void launch_onesweep(const SortJob& j, const PassParams& pp, hipStream_t s) {
     const uint32_t grid = (uint32_t)((j.num_tiles + round - 1) / round * round);
     if constexpr (!WIDE && THREADS >= 512) {
         if (j.lb_wide) {
-            hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, true>), dim3(grid), dim3(THREADS), 0, s, pp);
+            hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, true, false>), dim3(grid), dim3(THREADS), 0, s,
+                               pp);
             return;
         }
     }
-    hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, false>), dim3(grid), dim3(THREADS), 0, s, pp);
+    if constexpr (KB == 4 && VB == 4 && !WIDE) {
+        if (j.early_agg) {
+            hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, false, true>), dim3(grid), dim3(THREADS), 0, s,
+                               pp);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, false, false>), dim3(grid), dim3(THREADS), 0, s, pp);
 }
 
 template <int KB, int VB, bool WIDE>

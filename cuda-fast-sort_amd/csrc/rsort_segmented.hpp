@@ -472,6 +472,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_onesweep(SegParams a) {
     constexpr bool LB_GROUPS = false;  // each segment heads its own chain
     Off* lbg = nullptr;
     constexpr bool LB_SEGWALK = false;
+    constexpr bool EARLY_AGG = false;
     __shared__ OnesweepLds<KB, VB, WIDE, THREADS, K> L;
 
     const uint32_t pass = a.pass;

@@ -47,6 +47,12 @@ RSA_HD constexpr uint32_t tile_chunk(int) { return (uint32_t)RSA_XCD_CHUNK; }
 // bytes of the per-pass ticket counters at the start of the lookback region
 constexpr size_t kPartTicketBytes = 16 * kTicketParts * 4;  // 16 passes at most
 
+// u32/u32 passes of at least this many tiles publish each tile's aggregate
+// from digit counts taken before the rank (one LDS atomic per key more; the
+// successors then wait less).  Measured on MI355X: u32/u32 2^28 -2.9 %, but
+// 2^24 +4 %, and u32 keys, u64/u64 slower.
+constexpr uint64_t kEarlyAggMinTiles = 8192;
+
 // Passes of at most this many tiles use the one-round segmented lookback
 // walk (all of their tiles are resident at once on 256 CUs).
 constexpr uint64_t kLbWideMaxTiles = 512;

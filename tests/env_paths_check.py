@@ -1,9 +1,12 @@
-"""Parity of the persistent chunked pass kernel against the oracle.
+"""Parity against the oracle of the code paths that environment variables
+select: the persistent chunked pass kernel (RSORT_AMD_PASS_KERNEL=chunk),
+the early tile aggregate on every u32/u32 pass (RSORT_AMD_EARLY_AGG_TILES=1),
+the ordinary lookback walk at every size (RSORT_AMD_LB_WIDE_TILES=0) and the
+two-level histogram fold at every grid size (RSORT_AMD_HIST_DIRECT=0).
 
-The library reads RSORT_AMD_PASS_KERNEL once per process, so
-test_gpu_chunk_kernel.py runs this file in a process of its own with
-RSORT_AMD_PASS_KERNEL=chunk.  Cases: u32 keys and u32/u32 pairs in both
-orders, u64/u64 pairs, signed keys, and keys with an odd count of
+The library reads them once per process, so test_gpu_env_paths.py runs this
+file in a process of its own per setting.  Cases: u32 keys and u32/u32 pairs
+in both orders, u64/u64 pairs, signed keys, and keys with an odd count of
 non-uniform passes (the lowest uniform pass runs as a copy)."""
 from __future__ import annotations
 
@@ -26,7 +29,7 @@ from oracle import oracle as orc  # noqa: E402
 
 def check(keys, vals, kind, kb, vb, desc):
     n = keys.size // kb
-    print(f"[case] chunk kernel kind={kind} kb={kb} vb={vb} desc={int(desc)} n={n}", flush=True)
+    print(f"[case] {MODE} kind={kind} kb={kb} vb={vb} desc={int(desc)} n={n}", flush=True)
     dk = torch.from_numpy(keys.copy()).cuda()
     dv = torch.from_numpy(vals.copy()).cuda() if vals is not None else None
     d = rs.make_desc(kind, kb, vb, desc)
@@ -40,8 +43,11 @@ def check(keys, vals, kind, kb, vb, desc):
         assert np.array_equal(dv.cpu().numpy(), ev), "values differ"
 
 
+MODE = " ".join(f"{k}={v}" for k, v in sorted(os.environ.items()) if k.startswith("RSORT_AMD_") and k != "RSORT_AMD_LIB")
+
+
 def main():
-    assert os.environ.get("RSORT_AMD_PASS_KERNEL") == "chunk"
+    assert MODE, "set one of the RSORT_AMD_ path variables"
     assert torch.cuda.is_available()
     rs.lib()
     orc.lib()
@@ -49,12 +55,14 @@ def main():
     for n, desc in ((1000, False), (300_007, False), (300_007, True)):
         check(u8(random_keys(0, 4, n, seed=n)), None, 0, 4, 0, desc)
         check(u8(random_keys(0, 4, n, seed=n + 1)), u8(index_values(4, n)), 0, 4, 4, desc)
+    n = 1_000_003  # 62 histogram workgroups: more than one fold group
+    check(u8(random_keys(0, 4, n, seed=4)), u8(index_values(4, n)), 0, 4, 4, True)
     check(u8(random_keys(0, 8, 100_003, seed=5)), u8(index_values(8, 100_003)), 0, 8, 8, False)
     check(u8(random_keys(1, 4, 200_001, seed=6)), u8(index_values(4, 200_001)), 1, 4, 4, True)
     r = orc.mt19937(3, 200_000)
     k_odd = ((r & 0xFF000000) | 0x00123456).astype(np.uint32)
     check(u8(k_odd), u8(index_values(4, 200_000)), 0, 4, 4, False)
-    print("chunk kernel parity ok", flush=True)
+    print("env path parity ok", flush=True)
 
 
 if __name__ == "__main__":
