@@ -105,3 +105,22 @@ def test_u128_pairs_2p26(rs, orc):
     sort_dev(rs, keys, vals, 0, 16, 8, n)
     assert rs.check_pairings(orig, keys, vals, 16, 8, n) == 0
     assert torch.equal(torch.sort(vals)[0], torch.arange(n, dtype=torch.int64, device="cuda"))
+
+
+def test_u32_wide_offsets_2p29(rs, orc):
+    """2^29 + 7 keys: past 2^29 - 1 the lookback words and offsets are 64-bit
+    ("wide"); key-only and key-value, with the per-XCD tickets' partial last
+    round of tiles."""
+    n = (1 << 29) + 7
+    assert rs.plan(rs.make_desc(0, 4, 0), n).wide == 1
+    keys = torch.from_numpy(orc.mt19937(9, n).view(np.int32)).cuda()
+    expect = torch.sort(keys.to(torch.int64) & 0xFFFFFFFF)[0]
+    k1 = keys.clone()
+    sort_dev(rs, k1, None, 0, 4, 0, n)
+    assert torch.equal(k1.to(torch.int64) & 0xFFFFFFFF, expect)
+    del k1, expect
+    orig = keys.clone()
+    vals = torch.arange(n, dtype=torch.int32, device="cuda")
+    sort_dev(rs, keys, vals, 0, 4, 4, n)
+    assert rs.check_pairings(orig, keys, vals, 4, 4, n) == 0
+    assert torch.equal(torch.sort(vals)[0], torch.arange(n, dtype=torch.int32, device="cuda"))
