@@ -335,6 +335,8 @@ struct HistParams {
     uint32_t passes;
     uint32_t allow_skip;
     uint32_t direct;       // fold by device atomics from every workgroup (small grids)
+    uint4* self_clear;     // grid of one workgroup: it clears this region (control header, counts,
+    uint64_t self_clear_units;  // tickets, lookback) itself, in 16-byte units; no fill kernel runs
     TwMasks tw;
     unsigned long long* dbg;  // phase-timing slots (RSA_PHASE_TIMING builds), else null
 };
@@ -569,6 +571,17 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
 
     for (uint32_t i = tid; i < HWORDS; i += kHistThreads) s_hist[i] = 0u;
     if (tid == 0) s_skip = 0u;
+    if (a.self_clear_units) {
+        // the only workgroup: nothing else reads the control header, counts,
+        // tickets or lookback words before the passes, so it clears them here.
+        // The header goes through thread 0, which later stores the skip mask
+        // into it.
+        constexpr uint32_t hdr = (uint32_t)(sizeof(Ctrl) / 16u);
+        if (tid == 0)
+            for (uint32_t u = 0; u < hdr; ++u) a.self_clear[u] = make_uint4(0u, 0u, 0u, 0u);
+        for (uint64_t u = hdr + tid; u < a.self_clear_units; u += kHistThreads)
+            a.self_clear[u] = make_uint4(0u, 0u, 0u, 0u);
+    }
     __syncthreads();
 
 #if RSA_HIST_SLOTS
@@ -663,7 +676,25 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     Off* counts = (Off*)a.counts;
     const uint32_t groups = (gridDim.x + kHistGroup - 1) / kHistGroup;
     bool from_lds = false;  // the finalising workgroup holds the totals in LDS
-    if (a.direct) {
+    if (gridDim.x == 1u) {
+        // the only workgroup holds every count: totals into LDS, no fold
+        constexpr int PER = (P * 256 + kHistThreads - 1) / kHistThreads;
+        uint32_t t[PER];
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const uint32_t i = tid + (uint32_t)r * kHistThreads;
+            t[r] = i < (uint32_t)(P * 256) ? wg_total(i) : 0u;
+        }
+        __syncthreads();  // every slot read before the totals overwrite them
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const uint32_t i = tid + (uint32_t)r * kHistThreads;
+            if (i < (uint32_t)(P * 256)) s_hist[i] = t[r];
+        }
+        from_lds = true;
+        __syncthreads();
+        RSA_MARK(17);
+    } else if (a.direct) {
         // Small grids: every workgroup adds its counts to the global counts
         // with device atomics, and the last arrival finalises (three coherent
         // round trips fewer than the two-level fold below).

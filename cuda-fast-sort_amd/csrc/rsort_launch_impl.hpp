@@ -42,17 +42,20 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     constexpr uint32_t P = KB;
 
     record_event(j, s);
-    // 1. control header, global digit counts and lookback words (flag 0: never
-    //    ready in any epoch), contiguous in the workspace: one fill
+    // 1. control header, global digit counts, tickets and lookback words
+    //    (flag 0: never ready in any epoch), contiguous in the workspace: one
+    //    fill, or none when the histogram is a single workgroup (it clears
+    //    them itself)
     const uint64_t clear_units =
         (uint64_t)((sizeof(Ctrl) + (size_t)P * kRadix * sizeof(Off) + j.lb_bytes) / 16u);
-    hipLaunchKernelGGL(k_fill_zero16, dim3(grid_for(clear_units, 256u * 8u, j.num_cus, 4)), dim3(256), 0, s,
-                       (uint4*)j.ctrl, clear_units);
+    const uint32_t hg = grid_for(j.n, hist_keys_per_block_iter(KB), j.num_cus, j.hist_bpc ? j.hist_bpc : 1u);
+    const uint32_t hgrid = hg < kHistMaxGrid ? hg : kHistMaxGrid;
+    if (hgrid > 1u)
+        hipLaunchKernelGGL(k_fill_zero16, dim3(grid_for(clear_units, 256u * 8u, j.num_cus, 4)), dim3(256), 0, s,
+                           (uint4*)j.ctrl, clear_units);
     record_event(j, s);
 
     // 2. histogram of all passes, offsets, skip mask
-    const uint32_t hg = grid_for(j.n, hist_keys_per_block_iter(KB), j.num_cus, j.hist_bpc ? j.hist_bpc : 1u);
-    const uint32_t hgrid = hg < kHistMaxGrid ? hg : kHistMaxGrid;
     HistParams h{};
     h.keys = j.keys;
     h.partials = j.hist_partials;
@@ -69,6 +72,8 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     h.passes = P;
     h.allow_skip = j.allow_skip ? 1u : 0u;
     h.direct = (hgrid > (uint32_t)kHistGroup && hgrid <= j.hist_direct_max) ? 1u : 0u;
+    h.self_clear = hgrid == 1u ? (uint4*)j.ctrl : nullptr;
+    h.self_clear_units = hgrid == 1u ? clear_units : 0u;
     h.tw = j.tw;
     h.dbg = j.dbg;
     hipLaunchKernelGGL((k_histogram<KB, WIDE>), dim3(hgrid), dim3(kHistThreads), 0, s, h);
