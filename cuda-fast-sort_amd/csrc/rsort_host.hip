@@ -81,6 +81,16 @@ uint64_t early_agg_min_tiles() {
     return v;
 }
 
+// Per-XCD tile tickets of the pass kernel (where the device allows them).
+// RSORT_AMD_XCD_TICKETS=0 keeps the single global ticket everywhere.
+bool xcd_tickets_allowed() {
+    static const bool v = [] {
+        const char* e = getenv("RSORT_AMD_XCD_TICKETS");
+        return !(e && strcmp(e, "0") == 0);
+    }();
+    return v;
+}
+
 // Histogram grids of more than one fold group and at most this many
 // workgroups add their counts with device atomics directly instead of the
 // two-level fold.  RSORT_AMD_HIST_DIRECT overrides it (0 = never).
@@ -433,6 +443,7 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
     j.wide = p.wide;
     j.allow_skip = p.allow_skip;
     j.lb_wide = p.lb_wide;
+    j.xcd_tickets = xcd_tickets_allowed();
     j.early_agg = p.kb == 4 && p.vb_core == 4 && !p.wide && early_agg_min_tiles() > 0 &&
                   p.num_tiles >= early_agg_min_tiles();
     j.hist_direct_max = hist_direct_max();

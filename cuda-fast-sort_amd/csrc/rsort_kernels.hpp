@@ -349,7 +349,9 @@ struct PassParams {
     const void* gp;        // [passes][256] exclusive digit offsets
     void* lookback;        // [tiles][256] words
     void* lb_groups;       // [passes][groups][256] group words (kLbGroup tiles each), zeroed per sort
-    uint32_t* part_tickets;  // [passes][kTicketParts] tile ticket counters, zeroed per sort
+    uint32_t* tickets;     // this pass's tile ticket counters (tparts of them), zeroed per sort
+    uint32_t tparts;       // ticket counters: kTicketParts (one per XCD) or 1 (the global ticket)
+    uint32_t tchunk;       // consecutive tiles per ticket run (1 with the global ticket)
     Ctrl* ctrl;
     uint64_t n;
     uint32_t num_tiles;
@@ -1001,21 +1003,19 @@ __global__ __launch_bounds__(THREADS, pass_min_waves(KB, VB, K)) void k_onesweep
 #ifdef RSA_PHASE_TIMING
     if (threadIdx.x < 2) L.dbg[threadIdx.x] = 0u;
 #endif
-    constexpr uint32_t C = tile_chunk(KB);
-    uint32_t tile;
-    if constexpr (C > 0) {
-        // ticket k of counter `part` is tile (k / C) * (parts * C) + part * C + k % C;
-        // each counter hands out its tiles in increasing order, so a tile's
-        // predecessors have all been handed out before it.  The grid is rounded
-        // up to whole rounds of parts * C tiles: workgroups past the last tile
-        // end here.
-        const uint32_t part = blockIdx.x % kTicketParts;
-        const uint32_t tk = onesweep_begin(L, a.part_tickets + a.pass * kTicketParts + part);
-        tile = (tk / C) * (kTicketParts * C) + part * C + tk % C;
-        if (tile >= a.num_tiles) return;
-    } else {
-        tile = onesweep_begin(L, &a.ctrl->ticket[a.pass]);
-    }
+    // Ticket k of counter `part` (of a.tparts) is tile (k / C) * (parts * C)
+    // + part * C + k % C with C = a.tchunk.  One counter with C = 1 is the
+    // global ticket: tiles are handed out in increasing order, so a tile only
+    // ever waits on tiles already held by running workgroups.  With a counter
+    // per XCD each counter still hands out its tiles in increasing order, but
+    // a tile's predecessors from other counters may not be handed out yet:
+    // the host uses that only where the next parts * C workgroups in dispatch
+    // order can all be resident.  The grid is rounded up to whole rounds of
+    // parts * C tiles: workgroups past the last tile end here.
+    const uint32_t part = blockIdx.x % a.tparts;
+    const uint32_t tk = onesweep_begin(L, a.tickets + part);
+    const uint32_t tile = (tk / a.tchunk) * (a.tparts * a.tchunk) + part * a.tchunk + tk % a.tchunk;
+    if (tile >= a.num_tiles) return;
     TileGeom g;
     g.lb_tile = tile;
     g.head = tile == 0u;

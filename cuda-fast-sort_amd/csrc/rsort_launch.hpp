@@ -23,6 +23,7 @@ struct SortJob {
     bool allow_skip;      // skip passes whose digit is uniform
     bool lb_wide;         // one-round segmented lookback walk (passes of few tiles)
     bool early_agg;       // tile aggregates from digit counts taken before the rank
+    bool xcd_tickets;     // per-XCD tile tickets allowed (see launch_onesweep for when they are used)
     uint32_t hist_direct_max;  // histogram grids up to this fold by direct device atomics
     TwMasks tw;
     void* keys;           // caller buffer (in/out)
@@ -116,6 +117,15 @@ hipError_t launch_gather(void* dst, const void* src, const uint32_t* idx, uint64
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, int num_cus, hipStream_t s);
 hipError_t launch_extract(void* dst, const void* rec, uint64_t n, uint32_t record_bytes, uint32_t key_offset,
                           uint32_t key_bytes, int num_cus, hipStream_t s);
+
+// Resident workgroups of a kernel per CU (queried once per kernel; every
+// device this library runs on is gfx950).
+template <typename F>
+int blocks_per_cu(F kernel, int threads) {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, threads, 0) != hipSuccess || b < 1) b = 1;
+    return b;
+}
 
 inline uint32_t grid_for(uint64_t work, uint32_t per_block, int num_cus, uint32_t blocks_per_cu) {
     uint64_t g = (work + per_block - 1) / per_block;

@@ -7,30 +7,40 @@
 
 namespace rsa {
 
+// One pass launch of one kernel variant.  The per-XCD tile tickets make a
+// workgroup wait, in the worst case, on tiles held by the next
+// kTicketParts * C workgroups in dispatch order, which must then be able to
+// be resident at once: they are used only on a whole 8-XCD device (256 CUs)
+// where the kernel's resident workgroups number at least twice that;
+// otherwise the single global ticket, which never waits on a tile not yet
+// handed out.
+template <int KB, int VB, bool WIDE, int THREADS, int K, bool SEG, bool EARLY>
+void launch_pass_variant(const SortJob& j, PassParams pp, hipStream_t s) {
+    constexpr uint32_t C = tile_chunk(KB);
+    static const int occ = blocks_per_cu(k_onesweep<KB, VB, WIDE, THREADS, K, SEG, EARLY>, THREADS);
+    const bool xcd = C > 0 && j.xcd_tickets && j.num_cus >= 256 &&
+                     (uint64_t)occ * (uint64_t)j.num_cus >= 2ull * kTicketParts * C;
+    pp.tickets = xcd ? j.part_tickets + pp.pass * kTicketParts : &j.ctrl->ticket[pp.pass];
+    pp.tparts = xcd ? kTicketParts : 1u;
+    pp.tchunk = xcd ? C : 1u;
+    // whole rounds of the per-XCD ticket counters (extra workgroups end at once)
+    const uint64_t round = xcd ? (uint64_t)kTicketParts * C : 1u;
+    const uint32_t grid = (uint32_t)((j.num_tiles + round - 1) / round * round);
+    hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, SEG, EARLY>), dim3(grid), dim3(THREADS), 0, s, pp);
+}
+
 // One pass of the tile kernel; passes of few tiles take the variant with the
 // segmented lookback walk, large u32/u32 passes the one that publishes tile
 // aggregates before the rank.
 template <int KB, int VB, bool WIDE, int THREADS, int K>
 void launch_onesweep(const SortJob& j, const PassParams& pp, hipStream_t s) {
-    // whole rounds of the per-XCD ticket counters (extra workgroups end at once)
-    constexpr uint32_t C = tile_chunk(KB);
-    const uint64_t round = C ? (uint64_t)kTicketParts * C : 1u;
-    const uint32_t grid = (uint32_t)((j.num_tiles + round - 1) / round * round);
     if constexpr (!WIDE && THREADS >= 512) {
-        if (j.lb_wide) {
-            hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, true, false>), dim3(grid), dim3(THREADS), 0, s,
-                               pp);
-            return;
-        }
+        if (j.lb_wide) return launch_pass_variant<KB, VB, WIDE, THREADS, K, true, false>(j, pp, s);
     }
     if constexpr (KB == 4 && VB == 4 && !WIDE) {
-        if (j.early_agg) {
-            hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, false, true>), dim3(grid), dim3(THREADS), 0, s,
-                               pp);
-            return;
-        }
+        if (j.early_agg) return launch_pass_variant<KB, VB, WIDE, THREADS, K, false, true>(j, pp, s);
     }
-    hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, false, false>), dim3(grid), dim3(THREADS), 0, s, pp);
+    launch_pass_variant<KB, VB, WIDE, THREADS, K, false, false>(j, pp, s);
 }
 
 template <int KB, int VB, bool WIDE>
@@ -88,7 +98,6 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     pp.gp = j.gp;
     pp.lookback = j.lookback;
     pp.lb_groups = j.lb_groups;
-    pp.part_tickets = j.part_tickets;
     pp.ctrl = j.ctrl;
     pp.n = j.n;
     pp.num_tiles = (uint32_t)j.num_tiles;

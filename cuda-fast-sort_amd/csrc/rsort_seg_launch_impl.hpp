@@ -6,15 +6,6 @@
 
 namespace rsa {
 
-// Resident workgroups of a kernel per CU (queried once per kernel; every
-// device this library runs on is gfx950).
-template <typename F>
-int blocks_per_cu(F kernel, int threads) {
-    int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, threads, 0) != hipSuccess || b < 1) b = 1;
-    return b;
-}
-
 template <int KB, int VB, int BIN>
 hipError_t launch_seg_bin(const SegParams& a, const SegJob& j, hipStream_t s) {
     constexpr int THREADS = (int)seg_bin_threads(KB, VB, BIN);

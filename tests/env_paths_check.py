@@ -1,8 +1,9 @@
 """Parity against the oracle of the code paths that environment variables
 select: the persistent chunked pass kernel (RSORT_AMD_PASS_KERNEL=chunk),
 the early tile aggregate on every u32/u32 pass (RSORT_AMD_EARLY_AGG_TILES=1),
-the ordinary lookback walk at every size (RSORT_AMD_LB_WIDE_TILES=0) and the
-two-level histogram fold at every grid size (RSORT_AMD_HIST_DIRECT=0).
+the ordinary lookback walk at every size (RSORT_AMD_LB_WIDE_TILES=0), the
+two-level histogram fold at every grid size (RSORT_AMD_HIST_DIRECT=0) and the
+single global tile ticket (RSORT_AMD_XCD_TICKETS=0).
 
 The library reads them once per process, so test_gpu_env_paths.py runs this
 file in a process of its own per setting.  Cases: u32 keys and u32/u32 pairs

@@ -16,6 +16,7 @@ SETTINGS = {
     "early_aggregate_ordinary_walk_two_level_fold": {"RSORT_AMD_EARLY_AGG_TILES": "1",
                                                      "RSORT_AMD_LB_WIDE_TILES": "0",
                                                      "RSORT_AMD_HIST_DIRECT": "0"},
+    "global_tile_ticket": {"RSORT_AMD_XCD_TICKETS": "0"},
 }
 
 
