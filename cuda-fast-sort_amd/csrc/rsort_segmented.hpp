@@ -10,7 +10,8 @@
 //       segment -- three launches, no host round trip.
 //   k_seg_lds<bin>
 //       persistent: one workgroup takes the next segment of its bin from a
-//       ticket, loads it into registers, runs every LSD pass through LDS
+//       ticket (taking the following ticket while it sorts), loads the
+//       segment into registers, runs every LSD pass through LDS
 //       (rank in registers, exchange through one LDS staging buffer) and
 //       writes it back: two HBM touches per key.
 //   k_seg_large_init / k_seg_hist / k_seg_gp / k_seg_onesweep / k_seg_copy_back
@@ -219,18 +220,26 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
     __shared__ __attribute__((aligned(16))) uint8_t s_big[BIG];
     __shared__ __attribute__((aligned(16))) uint32_t s_cnt[WAVES * 256];
     __shared__ uint32_t s_wsum[4];
-    __shared__ uint32_t s_item;
+    __shared__ uint32_t s_item[2];  // by segment parity: this segment's ticket, the next one's
     __shared__ uint32_t s_uniform[2];  // by pass parity: reset two passes after its last read
 
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & 63u;
-    const uint32_t w = tid >> 6;
     const uint32_t count = DIRECT ? 0u : a.ctrl->cat_count[bin];
     const uint32_t* __restrict__ q = DIRECT ? nullptr : a.queue + a.ctrl->cat_base[bin];
-    const uint32_t wl = w * 64u * K + lane;  // tile-local index of item 0
     RSA_TIMER_START(a.dbg);
+    uint32_t next = 0;  // thread 0: the ticket of the workgroup's next segment
+    if constexpr (!DIRECT) {
+        if (threadIdx.x == 0) s_item[0] = atomicAdd(&a.ctrl->bin_ticket[bin], 1u);
+        __syncthreads();
+    }
 
     for (uint32_t round = 0;; ++round) {
+        // the thread index is re-read opaquely each segment, so that the
+        // compiler keeps no per-item indexes live across segments
+        uint32_t tid = threadIdx.x;
+        if constexpr (!DIRECT) asm volatile("" : "+v"(tid));
+        const uint32_t lane = tid & 63u;
+        const uint32_t w = tid >> 6;
+        const uint32_t wl = w * 64u * K + lane;  // tile-local index of item 0
         KM* __restrict__ keys;
         VM* __restrict__ vals;
         uint64_t sb;
@@ -243,15 +252,14 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
             sb = 0;
             m = (uint32_t)r.n;
         } else {
-            if (tid == 0) s_item = atomicAdd(&a.ctrl->bin_ticket[bin], 1u);
-            __syncthreads();
-            const uint32_t item = s_item;
+            // uniform: the segment's range and base pointers live in scalar registers
+            const uint32_t item = __builtin_amdgcn_readfirstlane(s_item[round & 1u]);
             if (item >= count) break;
             uint64_t se;
             seg_range(a, q[item], sb, se);
             m = (uint32_t)(se - sb);
-            keys = (KM*)a.keys;
-            vals = (VM*)a.vals;
+            keys = (KM*)a.keys + sb;
+            vals = VB > 0 ? (VM*)a.vals + sb : nullptr;
         }
         RSA_MARK(22);
 
@@ -262,7 +270,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
 #pragma unroll
         for (int i = 0; i < K; ++i) {
             const uint32_t idx = wl + (uint32_t)i * 64u;
-            const uint64_t at = sb + (idx < m ? idx : m - 1u);
+            const uint32_t at = idx < m ? idx : m - 1u;
             key[i] = key_to_reg<KB>(keys[at]);
             if constexpr (VB > 0) val[i] = vals[at];
         }
@@ -275,6 +283,8 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
         __builtin_amdgcn_s_waitcnt(0);
 #endif
         RSA_MARK(23);
+        // the next segment's ticket: its round trip overlaps this segment's passes
+        if (!DIRECT && tid == 0) next = atomicAdd(&a.ctrl->bin_ticket[bin], 1u);
 
         for (uint32_t p = 0; p < (uint32_t)KB; ++p) {
             // the previous exchange has been read back (barrier at its end)
@@ -353,10 +363,11 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
         for (int i = 0; i < K; ++i) {
             const uint32_t idx = wl + (uint32_t)i * 64u;
             if (idx < m) {
-                keys[sb + idx] = key_to_mem<KB>(tw_out<KB>(key[i], a.tw));
-                if constexpr (VB > 0) vals[sb + idx] = val[i];
+                keys[idx] = key_to_mem<KB>(tw_out<KB>(key[i], a.tw));
+                if constexpr (VB > 0) vals[idx] = val[i];
             }
         }
+        if (!DIRECT && tid == 0) s_item[(round + 1u) & 1u] = next;
         // s_item and the LDS tables are reused by the next segment
         __syncthreads();
         RSA_MARK(28);
