@@ -208,10 +208,14 @@ RSA_HD constexpr uint32_t seg_items(int key_bytes, int val_bytes) {
     return wide <= 4 ? 16u : (wide == 8 ? 8u : 4u);
 }
 
-// Bin b: threads and items per thread.  (Measured on MI355X: twice the
-// threads at half the items per bin, for more waves in flight, was slower.)
-RSA_HD constexpr uint32_t seg_bin_threads(int, int, int bin) { return 64u << bin; }
-RSA_HD constexpr uint32_t seg_bin_items(int key_bytes, int val_bytes, int) { return seg_items(key_bytes, val_bytes); }
+// Bin b: threads and items per thread.  Bin 0 runs two waves at half the
+// items: its one-wave segments were bound by the latency of each wave's rank
+// chain.  (Measured on MI355X: twice the threads at half the items in the
+// larger bins, which are bound by LDS throughput, was slower.)
+RSA_HD constexpr uint32_t seg_bin_threads(int, int, int bin) { return bin == 0 ? 128u : 64u << bin; }
+RSA_HD constexpr uint32_t seg_bin_items(int key_bytes, int val_bytes, int bin) {
+    return bin == 0 ? seg_items(key_bytes, val_bytes) / 2u : seg_items(key_bytes, val_bytes);
+}
 
 RSA_HD constexpr uint32_t seg_cap(int key_bytes, int val_bytes, int bin) {
     return seg_bin_items(key_bytes, val_bytes, bin) * seg_bin_threads(key_bytes, val_bytes, bin);
