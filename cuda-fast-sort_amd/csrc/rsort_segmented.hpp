@@ -211,7 +211,14 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
     constexpr size_t KSTAGE = (size_t)TILE * sizeof(KM);
     constexpr size_t VSTAGE = VB ? (size_t)TILE * sizeof(VM) : 0;
     constexpr size_t MATCH = (size_t)WAVES * 256 * sizeof(uint64_t);
-    constexpr size_t BIG = KSTAGE > VSTAGE ? (KSTAGE > MATCH ? KSTAGE : MATCH) : (VSTAGE > MATCH ? VSTAGE : MATCH);
+#ifndef RSA_SEG_PAIR_XCHG
+#define RSA_SEG_PAIR_XCHG 1
+#endif
+    // 4-byte keys with 4-byte values: one 64-bit exchange moves both, with
+    // half the exchange instructions and barriers (twice the staging LDS)
+    constexpr bool PAIR = RSA_SEG_PAIR_XCHG && KB == 4 && VB == 4;
+    constexpr size_t BIG0 = KSTAGE > VSTAGE ? (KSTAGE > MATCH ? KSTAGE : MATCH) : (VSTAGE > MATCH ? VSTAGE : MATCH);
+    constexpr size_t BIG = PAIR ? (KSTAGE + VSTAGE > MATCH ? KSTAGE + VSTAGE : MATCH) : BIG0;
     // digits owned per thread in the digit scan, and how many threads own digits
     constexpr int R = THREADS >= 256 ? 1 : 256 / THREADS;
     constexpr uint32_t OWNERS = THREADS >= 256 ? 256u : (uint32_t)THREADS;
@@ -339,9 +346,30 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
             __syncthreads();
             RSA_MARK(25);
             if (s_uniform[p & 1u]) continue;  // one digit: this pass keeps the order
+            if constexpr (PAIR) {
+                uint64_t* SP = (uint64_t*)s_big;
+#pragma unroll
+                for (int i = 0; i < K; ++i) {
+                    const uint32_t at = s_cnt[w * 256 + (dr[i] & 255u)] + (dr[i] >> 8);
+                    SP[at] = (uint64_t)(uint32_t)key_to_mem<KB>(key[i]) | ((uint64_t)(uint32_t)val[i] << 32);
+                }
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < K; ++i) {
+                    const uint64_t x = SP[wl + (uint32_t)i * 64u];
+                    key[i] = key_to_reg<KB>((KM)(uint32_t)x);
+                    val[i] = (VM)(uint32_t)(x >> 32);
+                }
+                __syncthreads();
+                continue;
+            }
             KM* SK = (KM*)s_big;
 #pragma unroll
-            for (int i = 0; i < K; ++i) SK[s_cnt[w * 256 + (dr[i] & 255u)] + (dr[i] >> 8)] = key_to_mem<KB>(key[i]);
+            for (int i = 0; i < K; ++i) {
+                const uint32_t at = s_cnt[w * 256 + (dr[i] & 255u)] + (dr[i] >> 8);
+                SK[at] = key_to_mem<KB>(key[i]);
+                if constexpr (VB > 0) dr[i] = at;  // the value goes to the same slot
+            }
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < K; ++i) key[i] = key_to_reg<KB>(SK[wl + (uint32_t)i * 64u]);
@@ -350,7 +378,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
                 __syncthreads();
                 VM* SV = (VM*)s_big;
 #pragma unroll
-                for (int i = 0; i < K; ++i) SV[s_cnt[w * 256 + (dr[i] & 255u)] + (dr[i] >> 8)] = val[i];
+                for (int i = 0; i < K; ++i) SV[dr[i]] = val[i];
                 __syncthreads();
 #pragma unroll
                 for (int i = 0; i < K; ++i) val[i] = SV[wl + (uint32_t)i * 64u];
