@@ -312,7 +312,8 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
 
             // digit scan: per digit, exclusive over waves; then exclusive over
             // digits; s_cnt[w][d] becomes the start of wave w's run of digit d
-            uint32_t tot[R], incl = 0, local = 0;
+            // the column's exclusive prefix stays in registers until the base is known
+            uint32_t tot[R], col[R][WAVES], incl = 0, local = 0;
             if (tid < OWNERS) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
@@ -320,9 +321,8 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
                     uint32_t s = 0;
 #pragma unroll
                     for (int ww = 0; ww < WAVES; ++ww) {
-                        const uint32_t c = s_cnt[ww * 256 + d];
-                        s_cnt[ww * 256 + d] = s;
-                        s += c;
+                        col[r][ww] = s;
+                        s += s_cnt[ww * 256 + d];
                     }
                     tot[r] = s;
                     local += s;
@@ -339,7 +339,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
                 for (int r = 0; r < R; ++r) {
                     const uint32_t d = tid * R + r;
 #pragma unroll
-                    for (int ww = 0; ww < WAVES; ++ww) s_cnt[ww * 256 + d] += base;
+                    for (int ww = 0; ww < WAVES; ++ww) s_cnt[ww * 256 + d] = base + col[r][ww];
                     base += tot[r];
                 }
             }
