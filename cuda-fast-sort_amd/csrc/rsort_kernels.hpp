@@ -832,6 +832,9 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
 // the key scatter), 1 with the keys, 2 after the key staging (overlapping
 // the lookback; measured fastest).  Kernels whose LDS allows two workgroups
 // per CU always use 0: their registers are budgeted for it.
+#ifndef RSA_SCAN_COL_REGS
+#define RSA_SCAN_COL_REGS 1
+#endif
 #ifndef RSA_VAL_AT
 #define RSA_VAL_AT 2
 #endif
