@@ -72,8 +72,38 @@ def gen_u32(n, device, seed):
     return (hi << 16) | lo  # uniform 32-bit patterns
 
 
+def timed_sorts(rs, desc, ks, vs, ws, tb, n, warmup, steps, profiled):
+    """warmup untimed sorts, then `steps` plain sorts back to back between two
+    synchronizations (wall ms per sort; device ms per sort from events between
+    the calls), then `profiled` sorts with per-stage events for the stage
+    breakdown.  Each sort takes its own fresh copy of the input."""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    val = lambda i: vs[i].data_ptr() if vs[i] is not None else None
+    for i in range(warmup):
+        rs.sort_raw(desc, ks[i].data_ptr(), val(i), ws.data_ptr(), tb, n, stream)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for j in range(steps):
+        i = warmup + j
+        rs.sort_raw(desc, ks[i].data_ptr(), val(i), ws.data_ptr(), tb, n, stream)
+        ev[j + 1].record(stream)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    dev_ms = [ev[j].elapsed_time(ev[j + 1]) for j in range(steps)]
+    stages = [rs.sort_profiled(desc, ks[i].data_ptr(), val(i), ws.data_ptr(), tb, n, stream)
+              for i in range(warmup + steps, warmup + steps + profiled)]
+    torch.cuda.synchronize()
+    return wall_ms, dev_ms, stages
+
+
 def run_sort_bench(rs, n, with_values, descending, steps, warmup, seed=1):
-    """Sorts steps+warmup fresh copies; returns (ms_per_step, stage ms list per timed step)."""
+    """Sorts warmup + steps + profiled fresh copies; returns (wall ms per timed
+    sort, stage ms lists of the profiled sorts, check, keys0, vals0, device ms
+    per timed sort)."""
     import torch
 
     dev = torch.device("cuda:0")
@@ -83,22 +113,12 @@ def run_sort_bench(rs, n, with_values, descending, steps, warmup, seed=1):
     desc = rs.make_desc(rs.KEY_UNSIGNED, 4, vb, descending)
     tb = rs.sort_raw(desc, None, None, None, 0, n)
     ws = torch.empty(tb, dtype=torch.uint8, device=dev)
-    total = steps + warmup
+    profiled = min(steps, 5)
+    total = warmup + steps + profiled
     ks = [keys0.clone() for _ in range(total)]
     vs = [vals0.clone() for _ in range(total)] if with_values else [None] * total
     torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    for i in range(warmup):
-        rs.sort_raw(desc, ks[i].data_ptr(), vs[i].data_ptr() if with_values else None, ws.data_ptr(), tb, n, stream)
-    torch.cuda.synchronize()
-    stages = []
-    t0 = time.perf_counter()
-    for i in range(warmup, total):
-        st = rs.sort_profiled(desc, ks[i].data_ptr(), vs[i].data_ptr() if with_values else None, ws.data_ptr(), tb,
-                              n, stream)
-        stages.append(st)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    wall_ms, dev_ms, stages = timed_sorts(rs, desc, ks, vs, ws, tb, n, warmup, steps, profiled)
     # spot check of the last result (sortedness + pairing) on the device
     k = ks[-1].view(torch.int32) ^ (-(2**31))  # signed order == unsigned order of the bits
     ok = bool((k[1:] <= k[:-1]).all().item()) if descending else bool((k[1:] >= k[:-1]).all().item())
@@ -109,7 +129,7 @@ def run_sort_bench(rs, n, with_values, descending, steps, warmup, seed=1):
         ok = ok and bool((vs[-1][1:][eq] > vs[-1][:-1][eq]).all().item())
     del ks, vs
     torch.cuda.empty_cache()
-    return (t1 - t0) * 1e3 / steps, stages, ok, keys0, vals0
+    return wall_ms, stages, ok, keys0, vals0, dev_ms
 
 
 def _fmix64(x):
@@ -175,26 +195,17 @@ def bench_config(rs, a):
     tb = rs.sort_raw(d, None, None, None, 0, n)
     ws = torch.empty(tb, dtype=torch.uint8, device=dev)
     before = rs.digit_counts(keys0, d, n)
-    total = a.steps + a.warmup
+    profiled = min(a.steps, 5)
+    total = a.warmup + a.steps + profiled
     ks = [keys0.clone() for _ in range(total)]
     vs = [vals0.clone() for _ in range(total)] if vb else [None] * total
     torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    for i in range(a.warmup):
-        rs.sort_raw(d, ks[i].data_ptr(), vs[i].data_ptr() if vb else None, ws.data_ptr(), tb, n, stream)
-    torch.cuda.synchronize()
-    stages = []
-    t0 = time.perf_counter()
-    for i in range(a.warmup, total):
-        stages.append(rs.sort_profiled(d, ks[i].data_ptr(), vs[i].data_ptr() if vb else None, ws.data_ptr(), tb, n,
-                                       stream))
-    torch.cuda.synchronize()
-    wall_ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    wall_ms, dev_steps, stages = timed_sorts(rs, d, ks, vs, ws, tb, n, a.warmup, a.steps, profiled)
     ok = rs.check_sorted(ks[-1], d, n) == 0 and torch.equal(rs.digit_counts(ks[-1], d, n), before)
     if vb:
         ok = ok and rs.check_pairings(keys0, ks[-1], vs[-1], kb, vb, n) == 0
     del ks, vs
-    dev_ms = sum(sum(s) for s in stages) / len(stages)
+    dev_ms = sum(dev_steps) / len(dev_steps)
     pass_ms = [sum(s[2 + p] for s in stages) / len(stages) for p in range(kb)]
     longest = max(pass_ms)
     executed = [p for p in range(kb) if pass_ms[p] > 0.05 * longest]
@@ -475,11 +486,11 @@ def main():
     kv = a.workload == "kv"
     kb, vb = 4, (4 if kv else 0)
     passes = 4
-    ms, stages, ok, keys0, _ = run_sort_bench(rs, n, kv, a.descending, a.steps, a.warmup)
-    # stage layout: [clear, histogram, pass0..pass3] (+ copy-back when present)
+    ms, stages, ok, keys0, _, dev_steps = run_sort_bench(rs, n, kv, a.descending, a.steps, a.warmup)
+    # stage layout (profiled sorts): [clear, histogram, pass0..pass3] (+ copy-back when present)
     pass_ms = [s[2 + p] for s in stages for p in range(passes)]
     hist_ms = [s[1] for s in stages]
-    sort_dev_ms = sum(sum(s) for s in stages) / len(stages)
+    sort_dev_ms = sum(dev_steps) / len(dev_steps)
     avg_pass = sum(pass_ms) / len(pass_ms)
     pass_bytes = 2.0 * n * (kb + vb)  # read + write of keys and values per pass
     achieved = pass_bytes / (avg_pass * 1e-3) / 1e9
@@ -524,8 +535,8 @@ def main():
     cgbs, _ = copy_bandwidth(n, 5)
     out["stream_copy_gbs"] = round(cgbs, 1)  # measured device copy bandwidth on this box
     if kv and not a.no_extra:
-        kms, kst, kok, _, _ = run_sort_bench(rs, n, False, a.descending, max(3, a.steps // 2), 2, seed=2)
-        kdev = sum(sum(s) for s in kst) / len(kst)
+        kms, kst, kok, _, _, kdevs = run_sort_bench(rs, n, False, a.descending, max(3, a.steps // 2), 2, seed=2)
+        kdev = sum(kdevs) / len(kdevs)
         kbytes = (1 + passes) * 2.0 * n * kb
         out["key_only"] = {
             "gkeys_per_s": round(n / (kms * 1e-3) / 1e9, 3), "ms_per_step": round(kms, 4),
