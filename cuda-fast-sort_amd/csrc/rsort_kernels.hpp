@@ -217,6 +217,9 @@ __device__ __forceinline__ T wave_incl_scan(T x) {
 // Compiler-only barrier: keeps the per-item LDS sequence of the ranker in
 // program order (the LDS executes one wave's instructions in order).
 #define RSA_COMPILER_FENCE() __atomic_signal_fence(__ATOMIC_SEQ_CST)
+#ifndef RSA_RANK_PIN
+#define RSA_RANK_PIN 1
+#endif
 
 // Phase timing (debug builds with -DRSA_PHASE_TIMING): thread 0 of each
 // workgroup adds the wall-clock ticks (100 MHz) of each phase to a slot of
@@ -297,6 +300,9 @@ __device__ __forceinline__ void rank_wave(const typename KeyTraits<KB>::Reg (&ke
         if (__all(d == d0)) {
             const uint32_t c = lds_ld(C + d0);
             rank = c + lane;
+#if RSA_RANK_PIN
+            asm volatile("" : "+v"(rank));
+#endif
             RSA_COMPILER_FENCE();
             if (lane == 0) lds_st(C + d0, (CT)(c + 64u));
         } else {
@@ -313,6 +319,12 @@ __device__ __forceinline__ void rank_wave(const typename KeyTraits<KB>::Reg (&ke
             const uint32_t c = lds_ld(C + d);
             const uint32_t below = mask_below(m);
             rank = c + below;
+#if RSA_RANK_PIN
+            // the rank is taken before the counter store is issued: otherwise
+            // the compiler computes it after the store and waits for the
+            // store's completion too, a second LDS round trip per item
+            asm volatile("" : "+v"(rank));
+#endif
             RSA_COMPILER_FENCE();
             if (below == 0u) lds_st(C + d, (CT)(c + (uint32_t)__builtin_popcountll(m)));
         }
