@@ -844,6 +844,9 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
 // the key scatter), 1 with the keys, 2 after the key staging (overlapping
 // the lookback; measured fastest).  Kernels whose LDS allows two workgroups
 // per CU always use 0: their registers are budgeted for it.
+#ifndef RSA_SCAN_BATCH
+#define RSA_SCAN_BATCH 2  // column words per LDS round trip in the pass tile's digit scan (4 spills at 64 VGPRs)
+#endif
 #ifndef RSA_SCAN_COL_REGS
 #define RSA_SCAN_COL_REGS 1
 #endif
