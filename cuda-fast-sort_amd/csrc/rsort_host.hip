@@ -48,17 +48,6 @@ size_t reserve(size_t* off, size_t bytes, size_t align = 256) {
 
 bool is_pow2_width(uint32_t b) { return b == 1 || b == 2 || b == 4 || b == 8 || b == 16; }
 
-// Pass kernel choice: RSORT_AMD_PASS_KERNEL=tile|chunk (default tile: on
-// MI355X the per-tile kernel measured faster -- the chunked kernel's single
-// workgroup per CU cannot hide its ranking latency).
-uint32_t pass_kernel_choice() {
-    static const uint32_t v = [] {
-        const char* e = getenv("RSORT_AMD_PASS_KERNEL");
-        return (e && strcmp(e, "chunk") == 0) ? (uint32_t)kPassChunk : (uint32_t)kPassTile;
-    }();
-    return v;
-}
-
 // Passes of at most this many tiles walk the lookback in segments, one round
 // of loads for up to (threads / 256 - 1) * 4 groups: there every tile starts
 // at once and the walks reach back to the first tiles before those have
@@ -208,7 +197,6 @@ TuneEntry tune_get(uint32_t kb, uint32_t vb_core) {
 }
 
 struct Plan {
-    uint32_t pass_kernel;
     uint32_t tile_cand, hist_bpc;
     uint32_t kind, kb, vb_user, vb_core;
     bool desc, allow_skip, index_mode, wide, lb_wide;
@@ -248,19 +236,13 @@ rsort_status make_plan(const rsort_desc* d, uint64_t n, Plan* p) {
         return fail(RSORT_ERROR_UNSUPPORTED, "values of %u bytes need n < 2^32", d->value_bytes);
     p->wide = n > kNarrowMaxN;
     p->passes = p->kb;
-    p->pass_kernel = pass_kernel_choice();
     const TuneEntry te = tune_get(p->kb, p->vb_core);
     const uint32_t cand = (te.tiny_log2n && n < (1ull << te.tiny_log2n))     ? te.tiny_tile_cand
                           : (te.small_log2n && n < (1ull << te.small_log2n)) ? te.small_tile_cand
                                                                              : te.tile_cand;
     p->tile_cand = tile_cands_built((int)p->kb, (int)p->vb_core) ? cand : 0u;
     p->hist_bpc = te.hist_bpc;
-    if (p->pass_kernel == kPassChunk) {
-        const ChunkCfg cc = chunk_cfg((int)p->kb, (int)p->vb_core);
-        p->cfg = TileCfg{cc.threads, cc.ipt * cc.subs};
-    } else {
-        p->cfg = tile_cand((int)p->kb, (int)p->vb_core, (int)p->tile_cand);
-    }
+    p->cfg = tile_cand((int)p->kb, (int)p->vb_core, (int)p->tile_cand);
     p->tile = (uint64_t)p->cfg.threads * p->cfg.items;
     p->num_tiles = n ? (n + p->tile - 1) / p->tile : 0;
     if (p->num_tiles > 0x7FFFFFFFull) return fail(RSORT_ERROR_UNSUPPORTED, "n too large");
@@ -366,6 +348,29 @@ rsort_status make_seg_plan(const rsort_desc* d, uint64_t n, uint32_t offset_byte
     return RSORT_SUCCESS;
 }
 
+// The per-XCD ticket lock of each device: one word, zero when free.  The
+// histogram of a sort whose passes may use per-XCD tile tickets takes it
+// with a device-scope compare-and-swap, and the last tile of its last
+// lookback pass clears it; a sort that finds it taken runs on the global
+// ticket.  The variable's name is only a host-side handle: kernels get its
+// device address, looked up once per device with that device current.
+static __device__ uint32_t rsa_xcd_lock;
+
+uint32_t* xcd_lock_word(int dev) {
+    static std::mutex mu;
+    static uint32_t* cache[64] = {nullptr};
+    static bool done[64] = {false};
+    if (dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!done[dev]) {
+        void* p = nullptr;
+        cache[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(rsa_xcd_lock)) == hipSuccess ? (uint32_t*)p : nullptr;
+        (void)hipGetLastError();
+        done[dev] = true;
+    }
+    return cache[dev];
+}
+
 int device_cus(int dev) {
     static std::mutex mu;
     static int cache[64] = {0};
@@ -435,7 +440,6 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
 
     uint8_t* base = (uint8_t*)(((uintptr_t)workspace + 255u) & ~(uintptr_t)255u);
     SortJob j{};
-    j.pass_kernel = p.pass_kernel;
     j.tile_cand = p.tile_cand;
     j.hist_bpc = p.hist_bpc;
     j.kb = p.kb;
@@ -444,6 +448,7 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
     j.allow_skip = p.allow_skip;
     j.lb_wide = p.lb_wide;
     j.xcd_tickets = xcd_tickets_allowed();
+    j.xcd_lock = j.xcd_tickets ? xcd_lock_word(dev) : nullptr;
     j.early_agg = p.kb == 4 && p.vb_core == 4 && !p.wide && early_agg_min_tiles() > 0 &&
                   p.num_tiles >= early_agg_min_tiles();
     j.hist_direct_max = hist_direct_max();

@@ -349,6 +349,8 @@ struct HistParams {
     uint32_t direct;       // fold by device atomics from every workgroup (small grids)
     uint4* self_clear;     // grid of one workgroup: it clears this region (control header, counts,
     uint64_t self_clear_units;  // tickets, lookback) itself, in 16-byte units; no fill kernel runs
+    uint32_t* xcd_lock;    // the device's per-XCD ticket lock word (null: never taken)
+    uint32_t want_xcd;     // the passes can run on per-XCD tickets (see k_onesweep)
     TwMasks tw;
     unsigned long long* dbg;  // phase-timing slots (RSA_PHASE_TIMING builds), else null
 };
@@ -361,9 +363,9 @@ struct PassParams {
     const void* gp;        // [passes][256] exclusive digit offsets
     void* lookback;        // [tiles][256] words
     void* lb_groups;       // [passes][groups][256] group words (kLbGroup tiles each), zeroed per sort
-    uint32_t* tickets;     // this pass's tile ticket counters (tparts of them), zeroed per sort
-    uint32_t tparts;       // ticket counters: kTicketParts (one per XCD) or 1 (the global ticket)
-    uint32_t tchunk;       // consecutive tiles per ticket run (1 with the global ticket)
+    uint32_t* tickets_xcd; // this pass's kTicketParts per-XCD tile ticket counters, zeroed per sort
+    uint32_t* xcd_lock;    // the device's per-XCD ticket lock word (released by the last walking pass)
+    uint32_t xcd_ok;       // per-XCD tickets allowed for this kernel (the grid is whole rounds of them)
     Ctrl* ctrl;
     uint64_t n;
     uint32_t num_tiles;
@@ -822,7 +824,28 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
         }
         __syncthreads();
     }
-    if (tid == 0) a.ctrl->skip_mask = a.allow_skip ? s_skip : 0u;
+    if (tid == 0) {
+        const uint32_t skip = a.allow_skip ? s_skip : 0u;
+        a.ctrl->skip_mask = skip;
+        // Per-XCD tile tickets can make a tile wait on a tile that no running
+        // workgroup holds yet (k_onesweep); of the sorts in flight on the
+        // device, only the one holding this lock uses them, so at most one
+        // pass kernel at a time depends on dispatch order.  Taken here, once
+        // per sort, when some pass walks the lookback; released by the last
+        // tile of the last such pass.
+        uint32_t mode = 0u;
+        if (a.want_xcd && a.xcd_lock) {
+            const uint32_t run = pass_run_mask(skip, (uint32_t)P);
+            if (run & ~skip) {
+                uint32_t expect = 0u;
+                mode = __hip_atomic_compare_exchange_strong(a.xcd_lock, &expect, 1u, __ATOMIC_RELAXED,
+                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           ? 1u
+                           : 0u;
+            }
+        }
+        a.ctrl->xcd_mode = mode;
+    }
     RSA_MARK(18);
     RSA_TOTAL(20);
 }
@@ -950,6 +973,7 @@ __device__ __forceinline__ uint32_t onesweep_begin(OnesweepLds<KB, VB, WIDE, THR
 // whether it is the first or last executed pass (twiddle in / out).
 struct PassRun {
     bool runs, copy, first, last;
+    bool last_walk;  // no later pass walks the lookback
     uint32_t e, epoch;
 };
 
@@ -963,6 +987,7 @@ __device__ __forceinline__ PassRun pass_run(uint32_t skip_mask, uint32_t passes,
     r.epoch = (uint32_t)__builtin_popcount(run & ~skip_mask & below) & 1u;
     r.first = r.e == 0u;
     r.last = (run >> (pass + 1u)) == 0u;
+    r.last_walk = ((run & ~skip_mask) >> (pass + 1u)) == 0u;
     return r;
 }
 
@@ -1021,18 +1046,28 @@ __global__ __launch_bounds__(THREADS, pass_min_waves(KB, VB, K)) void k_onesweep
 #ifdef RSA_PHASE_TIMING
     if (threadIdx.x < 2) L.dbg[threadIdx.x] = 0u;
 #endif
-    // Ticket k of counter `part` (of a.tparts) is tile (k / C) * (parts * C)
-    // + part * C + k % C with C = a.tchunk.  One counter with C = 1 is the
-    // global ticket: tiles are handed out in increasing order, so a tile only
-    // ever waits on tiles already held by running workgroups.  With a counter
-    // per XCD each counter still hands out its tiles in increasing order, but
-    // a tile's predecessors from other counters may not be handed out yet:
-    // the host uses that only where the next parts * C workgroups in dispatch
-    // order can all be resident.  The grid is rounded up to whole rounds of
-    // parts * C tiles: workgroups past the last tile end here.
-    const uint32_t part = blockIdx.x % a.tparts;
-    const uint32_t tk = onesweep_begin(L, a.tickets + part);
-    const uint32_t tile = (tk / a.tchunk) * (a.tparts * a.tchunk) + part * a.tchunk + tk % a.tchunk;
+    // Ticket k of counter `part` (of `parts`) is tile (k / C) * (parts * C)
+    // + part * C + k % C.  One counter with C = 1 is the global ticket: tiles
+    // are handed out in increasing order, so a tile only ever waits on tiles
+    // already held by running workgroups, whatever else runs on the device.
+    // With a counter per XCD (workgroup b is placed on XCD b mod 8, so each
+    // XCD works through runs of C neighbouring tiles, whose digit runs share
+    // partial cache lines) each counter still hands out its tiles in
+    // increasing order, but a tile's predecessors from other counters may not
+    // be handed out yet.  That needs the next parts * C workgroups in
+    // dispatch order to be resident together: the host allows it only where
+    // the kernel alone can hold twice that (xcd_ok), and of the sorts in
+    // flight on the device only the one holding the device's lock uses it
+    // (ctrl->xcd_mode, taken by the histogram), so no two pass kernels can
+    // each fill XCDs with workgroups waiting on the other's undispatched
+    // tiles.  The grid is whole rounds of parts * C tiles when xcd_ok:
+    // workgroups past the last tile end here.
+    const bool xcd = a.xcd_ok && a.ctrl->xcd_mode;
+    const uint32_t parts = xcd ? kTicketParts : 1u;
+    const uint32_t chunk = xcd ? tile_chunk(KB) : 1u;
+    const uint32_t part = blockIdx.x % parts;
+    const uint32_t tk = onesweep_begin(L, xcd ? a.tickets_xcd + part : &a.ctrl->ticket[a.pass]);
+    const uint32_t tile = ticket_tile(tk, part, parts, chunk);
     if (tile >= a.num_tiles) return;
     TileGeom g;
     g.lb_tile = tile;
@@ -1048,6 +1083,10 @@ __global__ __launch_bounds__(THREADS, pass_min_waves(KB, VB, K)) void k_onesweep
     const TwMasks& tw = a.tw;
     const uint32_t tile_tid = threadIdx.x;
 #include "rsort_onesweep_tile.inc"
+    // The last tile of the last pass that walks: every tile of the pass has
+    // been handed out, so the sort no longer depends on dispatch order.
+    if (xcd && pr.last_walk && tile + 1u == a.num_tiles && threadIdx.x == 0)
+        __hip_atomic_store(a.xcd_lock, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     RSA_TOTAL(10);
 }
 

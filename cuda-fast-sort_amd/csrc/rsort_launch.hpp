@@ -14,7 +14,6 @@
 namespace rsa {
 
 struct SortJob {
-    uint32_t pass_kernel; // PassKernel
     uint32_t tile_cand;   // tile_cand index of the pass kernel
     uint32_t hist_bpc;    // histogram workgroups per CU
     uint32_t kb;          // key bytes 1,2,4,8,16
@@ -23,7 +22,8 @@ struct SortJob {
     bool allow_skip;      // skip passes whose digit is uniform
     bool lb_wide;         // one-round segmented lookback walk (passes of few tiles)
     bool early_agg;       // tile aggregates from digit counts taken before the rank
-    bool xcd_tickets;     // per-XCD tile tickets allowed (see launch_onesweep for when they are used)
+    bool xcd_tickets;     // per-XCD tile tickets allowed (see pass_variant for when they are used)
+    uint32_t* xcd_lock;   // the device's per-XCD ticket lock word (device address; null: global ticket only)
     uint32_t hist_direct_max;  // histogram grids up to this fold by direct device atomics
     TwMasks tw;
     void* keys;           // caller buffer (in/out)

@@ -3,44 +3,64 @@
 
 #include "rsort_kernels.hpp"
 #include "rsort_launch.hpp"
-#include "rsort_onesweep_p.hpp"
 
 namespace rsa {
 
-// One pass launch of one kernel variant.  The per-XCD tile tickets make a
-// workgroup wait, in the worst case, on tiles held by the next
-// kTicketParts * C workgroups in dispatch order, which must then be able to
-// be resident at once: they are used only on a whole 8-XCD device (256 CUs)
-// where the kernel's resident workgroups number at least twice that;
-// otherwise the single global ticket, which never waits on a tile not yet
-// handed out.
+// The pass kernel variant of a sort, chosen once before the histogram (which
+// takes the device's per-XCD ticket lock only when the variant may use it).
+struct PassKernel {
+    const void* fn;
+    uint32_t threads;
+    uint32_t grid;
+    bool xcd_ok;
+};
+
+// Per-XCD tile tickets make a workgroup wait, in the worst case, on tiles
+// held by the next kTicketParts * C workgroups in dispatch order, which must
+// then be able to be resident at once: they are allowed only on a whole
+// 8-XCD device (256 CUs) where the kernel's resident workgroups number at
+// least twice that, and used only by the sort that holds the device's lock
+// (k_histogram); otherwise the single global ticket, which never waits on a
+// tile not yet handed out.
 template <int KB, int VB, bool WIDE, int THREADS, int K, bool SEG, bool EARLY>
-void launch_pass_variant(const SortJob& j, PassParams pp, hipStream_t s) {
+PassKernel pass_variant(const SortJob& j) {
     constexpr uint32_t C = tile_chunk(KB);
     static const int occ = blocks_per_cu(k_onesweep<KB, VB, WIDE, THREADS, K, SEG, EARLY>, THREADS);
-    const bool xcd = C > 0 && j.xcd_tickets && j.num_cus >= 256 &&
-                     (uint64_t)occ * (uint64_t)j.num_cus >= 2ull * kTicketParts * C;
-    pp.tickets = xcd ? j.part_tickets + pp.pass * kTicketParts : &j.ctrl->ticket[pp.pass];
-    pp.tparts = xcd ? kTicketParts : 1u;
-    pp.tchunk = xcd ? C : 1u;
+    PassKernel pk;
+    pk.fn = (const void*)k_onesweep<KB, VB, WIDE, THREADS, K, SEG, EARLY>;
+    pk.threads = THREADS;
+    pk.xcd_ok = C > 0 && j.xcd_tickets && j.xcd_lock && j.num_cus >= 256 &&
+                (uint64_t)occ * (uint64_t)j.num_cus >= 2ull * kTicketParts * C;
     // whole rounds of the per-XCD ticket counters (extra workgroups end at once)
-    const uint64_t round = xcd ? (uint64_t)kTicketParts * C : 1u;
-    const uint32_t grid = (uint32_t)((j.num_tiles + round - 1) / round * round);
-    hipLaunchKernelGGL((k_onesweep<KB, VB, WIDE, THREADS, K, SEG, EARLY>), dim3(grid), dim3(THREADS), 0, s, pp);
+    const uint64_t round = pk.xcd_ok ? (uint64_t)kTicketParts * C : 1u;
+    pk.grid = (uint32_t)((j.num_tiles + round - 1) / round * round);
+    return pk;
 }
 
-// One pass of the tile kernel; passes of few tiles take the variant with the
-// segmented lookback walk, large u32/u32 passes the one that publishes tile
-// aggregates before the rank.
+// Passes of few tiles take the variant with the segmented lookback walk,
+// large u32/u32 passes the one that publishes tile aggregates before the rank.
 template <int KB, int VB, bool WIDE, int THREADS, int K>
-void launch_onesweep(const SortJob& j, const PassParams& pp, hipStream_t s) {
+PassKernel pass_variant_of(const SortJob& j) {
     if constexpr (!WIDE && THREADS >= 512) {
-        if (j.lb_wide) return launch_pass_variant<KB, VB, WIDE, THREADS, K, true, false>(j, pp, s);
+        if (j.lb_wide) return pass_variant<KB, VB, WIDE, THREADS, K, true, false>(j);
     }
     if constexpr (KB == 4 && VB == 4 && !WIDE) {
-        if (j.early_agg) return launch_pass_variant<KB, VB, WIDE, THREADS, K, false, true>(j, pp, s);
+        if (j.early_agg) return pass_variant<KB, VB, WIDE, THREADS, K, false, true>(j);
     }
-    launch_pass_variant<KB, VB, WIDE, THREADS, K, false, false>(j, pp, s);
+    return pass_variant<KB, VB, WIDE, THREADS, K, false, false>(j);
+}
+
+// The tile candidate the tuning table chose (tile_cand).
+template <int KB, int VB, bool WIDE>
+PassKernel choose_pass_kernel(const SortJob& j) {
+    constexpr TileCfg C = tile_cfg(KB, VB);
+    if constexpr (tile_cands_built(KB, VB)) {
+        constexpr TileCfg C1 = tile_cand(KB, VB, 1), C2 = tile_cand(KB, VB, 2), C3 = tile_cand(KB, VB, 3);
+        if (j.tile_cand == 1) return pass_variant_of<KB, VB, WIDE, C1.threads, C1.items>(j);
+        if (j.tile_cand == 2) return pass_variant_of<KB, VB, WIDE, C2.threads, C2.items>(j);
+        if (j.tile_cand == 3) return pass_variant_of<KB, VB, WIDE, C3.threads, C3.items>(j);
+    }
+    return pass_variant_of<KB, VB, WIDE, C.threads, C.items>(j);
 }
 
 template <int KB, int VB, bool WIDE>
@@ -48,7 +68,6 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
     using KM = typename KeyTraits<KB>::Mem;
     using VM = typename ValTraits<VB>::Mem;
-    constexpr TileCfg C = tile_cfg(KB, VB);
     constexpr uint32_t P = KB;
 
     record_event(j, s);
@@ -84,6 +103,9 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     h.direct = (hgrid > (uint32_t)kHistGroup && hgrid <= j.hist_direct_max) ? 1u : 0u;
     h.self_clear = hgrid == 1u ? (uint4*)j.ctrl : nullptr;
     h.self_clear_units = hgrid == 1u ? clear_units : 0u;
+    const PassKernel pk = choose_pass_kernel<KB, VB, WIDE>(j);
+    h.xcd_lock = pk.xcd_ok ? j.xcd_lock : nullptr;
+    h.want_xcd = pk.xcd_ok ? 1u : 0u;
     h.tw = j.tw;
     h.dbg = j.dbg;
     hipLaunchKernelGGL((k_histogram<KB, WIDE>), dim3(hgrid), dim3(kHistThreads), 0, s, h);
@@ -104,26 +126,15 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     pp.passes = P;
     pp.tw = j.tw;
     pp.dbg = j.dbg;
-    constexpr ChunkCfg CC = chunk_cfg(KB, VB);
+    pp.tickets_xcd = j.part_tickets;
+    pp.xcd_lock = j.xcd_lock;
+    pp.xcd_ok = pk.xcd_ok ? 1u : 0u;
     for (uint32_t p = 0; p < P; ++p) {
         pp.pass = p;
-        if (j.pass_kernel == kPassChunk) {
-            // persistent: one workgroup per CU (its LDS allows no second)
-            const uint32_t grid = j.num_tiles < (uint64_t)j.num_cus ? (uint32_t)j.num_tiles : (uint32_t)j.num_cus;
-            hipLaunchKernelGGL((k_onesweep_p<KB, VB, WIDE, CC.threads, CC.ipt, CC.subs>), dim3(grid),
-                               dim3(CC.threads), 0, s, pp);
-        } else if (tile_cands_built(KB, VB) && j.tile_cand == 1) {
-            constexpr TileCfg C1 = tile_cand(KB, VB, 1);
-            launch_onesweep<KB, VB, WIDE, C1.threads, C1.items>(j, pp, s);
-        } else if (tile_cands_built(KB, VB) && j.tile_cand == 2) {
-            constexpr TileCfg C2 = tile_cand(KB, VB, 2);
-            launch_onesweep<KB, VB, WIDE, C2.threads, C2.items>(j, pp, s);
-        } else if (tile_cands_built(KB, VB) && j.tile_cand == 3) {
-            constexpr TileCfg C3 = tile_cand(KB, VB, 3);
-            launch_onesweep<KB, VB, WIDE, C3.threads, C3.items>(j, pp, s);
-        } else {
-            launch_onesweep<KB, VB, WIDE, C.threads, C.items>(j, pp, s);
-        }
+        pp.tickets_xcd = j.part_tickets + p * kTicketParts;
+        void* args[] = {&pp};
+        const hipError_t e = hipLaunchKernel(pk.fn, dim3(pk.grid), dim3(pk.threads), args, 0, s);
+        if (e != hipSuccess) return e;
         record_event(j, s);
     }
 

@@ -33,7 +33,8 @@ struct Ctrl {
     uint32_t hist_done;           // histogram workgroup groups finished
     uint32_t skip_mask;           // bit p set: every key has the same digit in pass p
     uint32_t group_done[32];      // histogram workgroups finished per group
-    uint32_t reserved[14];
+    uint32_t xcd_mode;            // 1: this sort holds the device's per-XCD ticket lock
+    uint32_t reserved[13];
 };
 static_assert(sizeof(Ctrl) == 256, "control header must stay 256 bytes");
 
@@ -404,6 +405,15 @@ RSA_HD W lookback_wide_finish(int64_t tile, uint32_t epoch, const W (&part)[NSEG
     }
     return lookback_walk_groups_from<W, WIN, GW>(acc, -1, 0, g - 1 - (int64_t)(NSEG - 1) * GW, epoch, load_tile,
                                                  load_group, backoff, steps, spins);
+}
+
+// ---------------------------------------------------------------------------
+// Tile tickets of a pass.  Ticket k of counter `part` (of `parts`, each
+// handing out runs of `chunk` consecutive tiles) is this tile; one counter
+// with chunk 1 is the global ticket (tile = k).
+// ---------------------------------------------------------------------------
+RSA_HD uint32_t ticket_tile(uint32_t k, uint32_t part, uint32_t parts, uint32_t chunk) {
+    return (k / chunk) * (parts * chunk) + part * chunk + k % chunk;
 }
 
 // ---------------------------------------------------------------------------

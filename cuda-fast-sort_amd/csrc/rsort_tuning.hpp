@@ -87,25 +87,6 @@ struct TileCfg {
     uint32_t items;
 };
 
-// Persistent chunked pass (k_onesweep_p): 1024 threads, chunks of SUBS
-// sub-tiles of threads*ipt keys; every sub-tile's keys (or values) are staged
-// at once in 96 KiB of LDS.
-struct ChunkCfg {
-    uint32_t threads;
-    uint32_t ipt;   // items per thread per sub-tile
-    uint32_t subs;  // sub-tiles per chunk
-};
-
-// Which pass kernel a sort uses: one workgroup per tile (k_onesweep) or the
-// persistent chunked kernel (k_onesweep_p).
-enum PassKernel : uint32_t { kPassTile = 0, kPassChunk = 1 };
-
-RSA_HD constexpr ChunkCfg chunk_cfg(int key_bytes, int val_bytes) {
-    const int wide = key_bytes > val_bytes ? key_bytes : val_bytes;
-    const uint32_t sub = wide <= 4 ? 8192u : (wide == 8 ? 4096u : 2048u);
-    return ChunkCfg{1024u, sub / 1024u, 3u};
-}
-
 // Tile of the pass kernel per key / value width: threads x items per thread.
 // Measured on MI355X: fewer, larger tiles win -- every tile adds a column of
 // 256 lookback words that its successors must read -- as long as the

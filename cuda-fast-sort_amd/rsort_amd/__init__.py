@@ -267,10 +267,34 @@ def _vbytes(values) -> int:
     return 0 if values is None else values.element_size() * (values[0].numel() if values.dim() > 1 else 1)
 
 
+def _check_tensor(t, name: str, need_bytes: int) -> None:
+    """A tensor handed to the library as a device buffer must be a contiguous
+    CUDA tensor holding at least need_bytes bytes (the library sees only its
+    address, so a smaller, strided or host tensor would be written out of
+    bounds)."""
+    if t is None:
+        return
+    if not getattr(t, "is_cuda", False):
+        raise ValueError(f"{name} must be a CUDA (device) tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    have = t.numel() * t.element_size()
+    if have < need_bytes:
+        raise ValueError(f"{name} holds {have} bytes, {need_bytes} needed")
+
+
 def _sort(keys, values, temp_bytes, workspace, n, stream, descending, key_type, flags):
     kt = key_type or key_type_of(keys.dtype)
-    desc = make_desc(kt.kind, kt.nbytes, _vbytes(values), descending, flags)
-    return sort_raw(desc, _ptr(keys), _ptr(values), _ptr(workspace), temp_bytes, int(n), stream)
+    vb = _vbytes(values)
+    desc = make_desc(kt.kind, kt.nbytes, vb, descending, flags)
+    n = int(n)
+    if workspace is not None:
+        if temp_bytes is None or int(temp_bytes) < 0:
+            raise ValueError("temp_bytes must be the workspace size")
+        _check_tensor(keys, "keys", n * kt.nbytes)
+        _check_tensor(values, "values", n * vb)
+        _check_tensor(workspace, "workspace", int(temp_bytes))
+    return sort_raw(desc, _ptr(keys), _ptr(values), _ptr(workspace), temp_bytes, n, stream)
 
 
 def sort(d_inout, temp_bytes, d_workspace, n, stream=None, key_type: KeyType | None = None, flags: int = 0) -> int:
@@ -339,8 +363,17 @@ def segmented_sort(keys, offsets, temp_bytes, d_workspace, values=None, descendi
     d_workspace None -> returns the workspace size (no device work).
     """
     kt = key_type or key_type_of(keys.dtype)
-    desc = make_desc(kt.kind, kt.nbytes, _vbytes(values), descending)
+    vb = _vbytes(values)
+    desc = make_desc(kt.kind, kt.nbytes, vb, descending)
     num_segments = max(int(offsets.numel()) - 1, 0)
+    if d_workspace is not None:
+        n = int(keys.shape[0])
+        if temp_bytes is None or int(temp_bytes) < 0:
+            raise ValueError("temp_bytes must be the workspace size")
+        _check_tensor(keys, "keys", n * kt.nbytes)
+        _check_tensor(values, "values", n * vb)
+        _check_tensor(offsets, "offsets", (num_segments + 1) * offsets.element_size())
+        _check_tensor(d_workspace, "workspace", int(temp_bytes))
     return segmented_sort_raw(desc, _ptr(keys), _ptr(values), int(keys.shape[0]), _ptr(offsets),
                               offsets.element_size(), num_segments, _ptr(d_workspace), temp_bytes, stream)
 

@@ -1,6 +1,5 @@
 """Parity against the oracle of the code paths that environment variables
-select: the persistent chunked pass kernel (RSORT_AMD_PASS_KERNEL=chunk),
-the early tile aggregate on every u32/u32 pass (RSORT_AMD_EARLY_AGG_TILES=1),
+select: the early tile aggregate on every u32/u32 pass (RSORT_AMD_EARLY_AGG_TILES=1),
 the ordinary lookback walk at every size (RSORT_AMD_LB_WIDE_TILES=0), the
 two-level histogram fold at every grid size (RSORT_AMD_HIST_DIRECT=0) and the
 single global tile ticket (RSORT_AMD_XCD_TICKETS=0).

@@ -12,7 +12,6 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 SETTINGS = {
-    "chunked_pass_kernel": {"RSORT_AMD_PASS_KERNEL": "chunk"},
     "early_aggregate_ordinary_walk_two_level_fold": {"RSORT_AMD_EARLY_AGG_TILES": "1",
                                                      "RSORT_AMD_LB_WIDE_TILES": "0",
                                                      "RSORT_AMD_HIST_DIRECT": "0"},
