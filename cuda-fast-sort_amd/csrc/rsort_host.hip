@@ -412,33 +412,43 @@ hipError_t dispatch_core(const SortJob& j, hipStream_t s) {
     }
 }
 
-rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* workspace, size_t* temp_bytes, uint64_t n,
-                      void* stream, hipEvent_t* events, int max_events, int* n_events) {
-    Plan p;
-    rsort_status st = make_plan(d, n, &p);
-    if (st != RSORT_SUCCESS) return st;
-    if (!temp_bytes) return fail(RSORT_ERROR_INVALID_ARGUMENT, "temp_bytes is NULL");
-    if (!workspace) {
-        *temp_bytes = p.total;
-        return RSORT_SUCCESS;
-    }
-    if (*temp_bytes < p.total)
-        return fail(RSORT_ERROR_WORKSPACE_TOO_SMALL, "workspace of %zu bytes, %zu required", *temp_bytes, p.total);
-    if (n <= 1) return RSORT_SUCCESS;
+// Control region of a plan: the control header, digit counts, tile tickets,
+// lookback words, digit offsets and histogram partials -- everything from
+// off_ctrl to the end of the layout -- in bytes.
+size_t plan_ctrl_bytes(const Plan& p) { return p.total - 256u - p.off_ctrl; }
+
+// Where one sort's buffers live: the ping-pong partners of the keys and
+// values, the control region (plan offset off_ctrl maps to ctrl_region) and,
+// for value widths moved by index, the workspace holding the index and the
+// value copy (plan offsets off_idx, off_vals_copy from index_base).
+struct SortBuffers {
+    void* tmp_keys;
+    void* tmp_vals;
+    uint8_t* ctrl_region;
+    uint8_t* index_base;
+};
+
+// Argument checks shared by the entry points (after the plan, before device work).
+rsort_status check_sort_args(const Plan& p, const void* keys, const void* values) {
     if (!keys) return fail(RSORT_ERROR_INVALID_ARGUMENT, "keys is NULL");
     if (p.vb_user && !values) return fail(RSORT_ERROR_INVALID_ARGUMENT, "values is NULL but value_bytes = %u", p.vb_user);
     if ((uintptr_t)keys % p.kb) return fail(RSORT_ERROR_INVALID_ARGUMENT, "keys not aligned to %u bytes", p.kb);
     if (p.vb_user && is_pow2_width(p.vb_user) && (uintptr_t)values % p.vb_user)
         return fail(RSORT_ERROR_INVALID_ARGUMENT, "values not aligned to %u bytes", p.vb_user);
-    if ((uintptr_t)workspace % 16) return fail(RSORT_ERROR_INVALID_ARGUMENT, "workspace not 16-byte aligned");
+    return RSORT_SUCCESS;
+}
 
+// Enqueues one sort of plan p on `stream` (n >= 2, arguments checked).
+rsort_status enqueue_sort(const Plan& p, void* keys, void* values, const SortBuffers& b, uint64_t n, void* stream,
+                          unsigned long long* dbg, hipEvent_t* events, int max_events, int* n_events) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
     const int cus = device_cus(dev);
     hipStream_t s = (hipStream_t)stream;
+    // plan offsets from off_ctrl on map into the control region
+    auto at = [&](size_t off) { return b.ctrl_region + (off - p.off_ctrl); };
 
-    uint8_t* base = (uint8_t*)(((uintptr_t)workspace + 255u) & ~(uintptr_t)255u);
     SortJob j{};
     j.tile_cand = p.tile_cand;
     j.hist_bpc = p.hist_bpc;
@@ -454,22 +464,22 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
     j.hist_direct_max = hist_direct_max();
     j.tw = make_tw_masks(p.kind, p.kb, p.desc);
     j.keys = keys;
-    j.tmp_keys = base + p.off_tmp_keys;
-    j.tmp_vals = p.vb_core ? base + p.off_tmp_vals : nullptr;
-    j.vals = p.index_mode ? (void*)(base + p.off_idx) : values;
-    j.ctrl = (Ctrl*)(base + p.off_ctrl);
-    j.counts = base + p.off_counts;
-    j.gp = base + p.off_gp;
-    j.hist_partials = (uint32_t*)(base + p.off_hist_partials);
-    j.part_tickets = (uint32_t*)(base + p.off_lb);
-    j.lookback = base + p.off_lb + kPartTicketBytes;
+    j.tmp_keys = b.tmp_keys;
+    j.tmp_vals = p.vb_core ? b.tmp_vals : nullptr;
+    j.vals = p.index_mode ? (void*)(b.index_base + p.off_idx) : values;
+    j.ctrl = (Ctrl*)at(p.off_ctrl);
+    j.counts = at(p.off_counts);
+    j.gp = at(p.off_gp);
+    j.hist_partials = (uint32_t*)at(p.off_hist_partials);
+    j.part_tickets = (uint32_t*)at(p.off_lb);
+    j.lookback = at(p.off_lb) + kPartTicketBytes;
     j.lb_bytes = p.lb_bytes;
-    j.lb_groups = base + p.off_lb + p.lb_group_off;
+    j.lb_groups = at(p.off_lb) + p.lb_group_off;
     j.n = n;
     j.num_tiles = p.num_tiles;
     j.passes = p.passes;
     j.num_cus = cus;
-    j.dbg = debug_slots();
+    j.dbg = dbg;
     j.events = events;
     j.max_events = max_events;
     j.n_events = n_events;
@@ -481,12 +491,103 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
     e = dispatch_core(j, s);
     if (e != hipSuccess) return hip_fail(e, "sort launch");
     if (p.index_mode) {
-        void* copy = base + p.off_vals_copy;
+        void* copy = b.index_base + p.off_vals_copy;
         e = launch_gather(copy, values, (const uint32_t*)j.vals, n, p.vb_user, cus, s);
         if (e == hipSuccess) e = launch_copy(values, copy, n * p.vb_user, cus, s);
         if (e != hipSuccess) return hip_fail(e, "value gather");
     }
     return RSORT_SUCCESS;
+}
+
+rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* workspace, size_t* temp_bytes, uint64_t n,
+                      void* stream, hipEvent_t* events, int max_events, int* n_events) {
+    Plan p;
+    rsort_status st = make_plan(d, n, &p);
+    if (st != RSORT_SUCCESS) return st;
+    if (!temp_bytes) return fail(RSORT_ERROR_INVALID_ARGUMENT, "temp_bytes is NULL");
+    if (!workspace) {
+        *temp_bytes = p.total;
+        return RSORT_SUCCESS;
+    }
+    if (*temp_bytes < p.total)
+        return fail(RSORT_ERROR_WORKSPACE_TOO_SMALL, "workspace of %zu bytes, %zu required", *temp_bytes, p.total);
+    if (n <= 1) return RSORT_SUCCESS;
+    st = check_sort_args(p, keys, values);
+    if (st != RSORT_SUCCESS) return st;
+    if ((uintptr_t)workspace % 16) return fail(RSORT_ERROR_INVALID_ARGUMENT, "workspace not 16-byte aligned");
+
+    uint8_t* base = (uint8_t*)(((uintptr_t)workspace + 255u) & ~(uintptr_t)255u);
+    SortBuffers b{};
+    b.tmp_keys = base + p.off_tmp_keys;
+    b.tmp_vals = p.vb_core ? base + p.off_tmp_vals : nullptr;
+    b.ctrl_region = base + p.off_ctrl;
+    b.index_base = base;
+    return enqueue_sort(p, keys, values, b, n, stream, debug_slots(), events, max_events, n_events);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free sort: the control region lives in a file-scope device
+// array instead of a caller's workspace, and the ping-pong partners are the
+// caller's own buffers.  Every device the library runs on holds its own copy
+// of the array (kScratchBytes) until the process ends.  The array's name is
+// a host-side handle only: kernels and copies get its device address, looked
+// up once per device with that device current.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+uint8_t* scratch_ptr(int dev) {
+    static std::mutex mu;
+    static uint8_t* cache[64] = {nullptr};
+    if (dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!cache[dev]) {
+        uint8_t* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        cache[dev] = ptr;
+    }
+    return cache[dev];
+}
+
+rsort_status run_sort_noalloc(const rsort_desc* d, void* keys, void* values, void* keys_alt, void* values_alt,
+                              uint64_t n, void* stream) {
+    Plan p;
+    rsort_status st = make_plan(d, n, &p);
+    if (st != RSORT_SUCCESS) return st;
+    if (p.index_mode)
+        return fail(RSORT_ERROR_UNSUPPORTED,
+                    "the allocation-free sort takes value widths 0, 1, 2, 4, 8 and 16 bytes, not %u", p.vb_user);
+    const size_t need = plan_ctrl_bytes(p);
+    if (need > kScratchBytes)
+        return fail(RSORT_ERROR_UNSUPPORTED,
+                    "n = %llu needs %zu bytes of sort state; the allocation-free sort holds %zu per device",
+                    (unsigned long long)n, need, kScratchBytes);
+    if (n <= 1) return RSORT_SUCCESS;
+    st = check_sort_args(p, keys, values);
+    if (st != RSORT_SUCCESS) return st;
+    if (!keys_alt) return fail(RSORT_ERROR_INVALID_ARGUMENT, "keys_alt is NULL");
+    if (p.vb_core && !values_alt) return fail(RSORT_ERROR_INVALID_ARGUMENT, "values_alt is NULL but value_bytes = %u",
+                                              p.vb_user);
+    if ((uintptr_t)keys_alt % p.kb) return fail(RSORT_ERROR_INVALID_ARGUMENT, "keys_alt not aligned to %u bytes", p.kb);
+    if (p.vb_core && (uintptr_t)values_alt % p.vb_core)
+        return fail(RSORT_ERROR_INVALID_ARGUMENT, "values_alt not aligned to %u bytes", p.vb_core);
+    if (keys_alt == keys || (p.vb_core && values_alt == values))
+        return fail(RSORT_ERROR_INVALID_ARGUMENT, "the alternate buffers must be distinct from keys and values");
+    int dev = 0;
+    const hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    uint8_t* sp = scratch_ptr(dev);
+    if (!sp) return fail(RSORT_ERROR_DEVICE, "hipGetSymbolAddress of the sort state failed on device %d", dev);
+    SortBuffers b{};
+    b.tmp_keys = keys_alt;
+    b.tmp_vals = p.vb_core ? values_alt : nullptr;
+    b.ctrl_region = sp;
+    b.index_base = nullptr;
+    return enqueue_sort(p, keys, values, b, n, stream, nullptr, nullptr, 0, nullptr);
 }
 
 hipError_t dispatch_seg(const SegJob& j, hipStream_t s) {
@@ -648,6 +749,11 @@ extern "C" {
 rsort_status rsort_sort(const rsort_desc* desc, void* keys, void* values, void* workspace, size_t* temp_bytes,
                         uint64_t n, void* stream) {
     return rsa::run_sort(desc, keys, values, workspace, temp_bytes, n, stream, nullptr, 0, nullptr);
+}
+
+rsort_status rsort_sort_noalloc(const rsort_desc* desc, void* keys, void* values, void* keys_alt, void* values_alt,
+                                uint64_t n, void* stream) {
+    return rsa::run_sort_noalloc(desc, keys, values, keys_alt, values_alt, n, stream);
 }
 
 // Records sorted by one key field: the keys are copied out of the records

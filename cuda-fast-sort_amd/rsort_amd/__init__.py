@@ -125,6 +125,8 @@ def lib() -> ctypes.CDLL:
         vp, u64, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
         L.rsort_sort.argtypes = [ctypes.POINTER(Desc), vp, vp, vp, ctypes.POINTER(ctypes.c_size_t), u64, vp]
         L.rsort_sort.restype = i32
+        L.rsort_sort_noalloc.argtypes = [ctypes.POINTER(Desc), vp, vp, vp, vp, u64, vp]
+        L.rsort_sort_noalloc.restype = i32
         L.rsort_sort_profiled.argtypes = [ctypes.POINTER(Desc), vp, vp, vp, ctypes.POINTER(ctypes.c_size_t), u64, vp,
                                           ctypes.POINTER(ctypes.c_float), i32, ctypes.POINTER(i32)]
         L.rsort_sort_profiled.restype = i32
@@ -227,6 +229,15 @@ def sort_raw(desc: Desc, keys_ptr: int | None, vals_ptr: int | None, ws_ptr: int
     tb = ctypes.c_size_t(temp_bytes or 0)
     _check(lib().rsort_sort(ctypes.byref(desc), keys_ptr, vals_ptr, ws_ptr, ctypes.byref(tb), n, _stream_ptr(stream)))
     return int(tb.value)
+
+
+def sort_noalloc_raw(desc: Desc, keys_ptr: int | None, vals_ptr: int | None, keys_alt_ptr: int | None,
+                     vals_alt_ptr: int | None, n: int, stream=None) -> None:
+    """Direct call of rsort_sort_noalloc: sorts in place with the caller's
+    alternate buffers as ping-pong partners and the library's per-device sort
+    state; allocates nothing."""
+    _check(lib().rsort_sort_noalloc(ctypes.byref(desc), keys_ptr, vals_ptr, keys_alt_ptr, vals_alt_ptr, n,
+                                    _stream_ptr(stream)))
 
 
 def sort_records_raw(desc: Desc, records_ptr: int | None, n: int, record_bytes: int, key_offset: int,

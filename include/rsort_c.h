@@ -77,6 +77,26 @@ rsort_status rsort_sort(const rsort_desc* desc, void* keys, void* values, void* 
                         uint64_t n, void* stream);
 
 /*
+ * Allocation-free sort.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory or hand over a workspace at call time
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * allocation).  Same result as rsort_sort (stable, in place in keys /
+ * values); keys_alt and values_alt are the caller's buffers of n keys and n
+ * values (values_alt ignored without values), overwritten as the ping-pong
+ * partners that rsort_sort keeps in its workspace.  The rest of the sort's
+ * state (control header, digit counts, tile tickets, lookback words, digit
+ * offsets) lives in a 32 MiB device array of the library, one per device,
+ * held until the process ends; nothing is allocated per call.  Sorts whose
+ * state exceeds it return RSORT_ERROR_UNSUPPORTED before any device work
+ * (it holds u32 keys or u32/u32 pairs up to 2^28 keys, u64/u64 up to 2^26,
+ * u128/u64 up to 2^24); so do value widths other than 0, 1, 2, 4, 8, 16.
+ * Every call on one device shares that array: calls that overlap on a device
+ * (two streams, two host threads) race and must be serialised by the caller.
+ */
+rsort_status rsort_sort_noalloc(const rsort_desc* desc, void* keys, void* values, void* keys_alt, void* values_alt,
+                                uint64_t n, void* stream);
+
+/*
  * Same as rsort_sort, then synchronises the stream and reports the device
  * time of each launched stage in milliseconds, in launch order:
  *   [0] control-block clear, [1] histogram, [2..2+P) one per pass,

@@ -373,3 +373,28 @@ def test_tensor_wrappers_reject_bad_buffers(rs):
         rs.sort(Dev(torch.zeros(200, dtype=torch.int32)[::2]), 1 << 20, Dev(ws), 100)
     with pytest.raises(ValueError, match="offsets must be contiguous"):
         rs.segmented_sort(Dev(keys), Dev(torch.tensor([0, 9, 50, 9, 100], dtype=torch.int32)[::2]), 1 << 20, Dev(ws))
+
+
+def test_noalloc_rejects_before_device_work(rs):
+    """The allocation-free entry point refuses, before any device work, sorts
+    whose state exceeds its per-device array and value widths moved by index
+    (the pointers here are never dereferenced)."""
+    fake = 0x10000
+    ok_sizes = [((4, 4), 1 << 28), ((8, 8), 1 << 26), ((16, 8), 1 << 24)]
+    too_big = [((4, 4), 1 << 29), ((8, 8), 1 << 27), ((16, 8), 1 << 25)]
+    for (kb, vb), n in too_big:
+        with pytest.raises(rs.RsortError) as e:
+            rs.sort_noalloc_raw(rs.make_desc(0, kb, vb), fake, fake, fake * 2, fake * 2, n)
+        assert e.value.status == 2 and "allocation-free" in rs.last_error()
+    with pytest.raises(rs.RsortError) as e:
+        rs.sort_noalloc_raw(rs.make_desc(0, 4, 12), fake, fake, fake * 2, fake * 2, 1000)
+    assert e.value.status == 2
+    # argument errors also come before device work
+    for args in [(0, fake, fake * 2, fake * 2), (fake, 0, fake * 2, fake * 2), (fake, fake, 0, fake * 2),
+                 (fake, fake, fake * 2, 0), (fake, fake, fake, fake * 2)]:
+        with pytest.raises(rs.RsortError) as e:
+            rs.sort_noalloc_raw(rs.make_desc(0, 4, 4), *args, 1000)
+        assert e.value.status == 1
+    # trivial sizes are no-ops that touch nothing
+    for (kb, vb), n in ok_sizes:
+        rs.sort_noalloc_raw(rs.make_desc(0, kb, vb), fake, fake, fake * 2, fake * 2, 1)
