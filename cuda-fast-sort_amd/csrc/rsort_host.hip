@@ -16,6 +16,7 @@
 #include <dlfcn.h>
 
 #include <mutex>
+#include <vector>
 
 #include "../../include/rsort_c.h"
 #include "rsort_kernels.hpp"
@@ -217,7 +218,8 @@ rsort_status check_desc(const rsort_desc* d) {
     if (d->key_kind == RSORT_KEY_FLOAT && d->key_bytes == 1)
         return fail(RSORT_ERROR_UNSUPPORTED, "no 1-byte floating key type");
     if (d->descending > 1) return fail(RSORT_ERROR_INVALID_ARGUMENT, "descending must be 0 or 1");
-    if (d->flags & ~RSORT_FLAG_NO_PASS_SKIP) return fail(RSORT_ERROR_INVALID_ARGUMENT, "unknown flags 0x%x", d->flags);
+    if (d->flags & ~(RSORT_FLAG_NO_PASS_SKIP | RSORT_FLAG_GRAPH))
+        return fail(RSORT_ERROR_INVALID_ARGUMENT, "unknown flags 0x%x", d->flags);
     return RSORT_SUCCESS;
 }
 
@@ -412,6 +414,122 @@ hipError_t dispatch_core(const SortJob& j, hipStream_t s) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Graph replay (RSORT_FLAG_GRAPH).  The first call with a given set of
+// arguments sorts directly and then captures the same enqueue, on a private
+// stream of the device, into an executable graph; later calls with the same
+// arguments replay it with one hipGraphLaunch on the caller's stream.  Every
+// launch of the sequence is a kernel (the clears are fill kernels), so the
+// graph holds kernel nodes only.  Up to kGraphCacheMax graphs are kept; the
+// least recently used one is destroyed after its last replay has completed.
+// (The reference captures and instantiates its enqueue on every call,
+// radix_kernel.cuh:583-600.)
+// ---------------------------------------------------------------------------
+struct GraphKey {
+    int dev;
+    uint32_t entry;  // 0: rsort_sort, 1: rsort_segmented_sort
+    rsort_desc desc;
+    uint64_t keys, values, workspace, offsets;
+    uint64_t n, num_segments, temp_bytes;
+    uint32_t offset_bytes, pad;
+};
+
+struct GraphEntry {
+    GraphKey key;
+    hipGraphExec_t exec;  // null: the capture failed; these arguments run directly
+    hipEvent_t done;      // recorded after every replay
+    uint64_t used;
+};
+
+constexpr size_t kGraphCacheMax = 16;
+std::mutex g_graph_mu;
+
+std::vector<GraphEntry>& graph_cache() {
+    static std::vector<GraphEntry>* v = new std::vector<GraphEntry>();  // kept until the process ends
+    return *v;
+}
+
+hipStream_t capture_stream(int dev) {
+    static hipStream_t cs[64] = {nullptr};
+    if (dev < 0 || dev >= 64) return nullptr;
+    if (!cs[dev] && hipStreamCreateWithFlags(&cs[dev], hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        cs[dev] = nullptr;
+    }
+    return cs[dev];
+}
+
+// `enqueue(s)` enqueues the whole sort on stream s; caller holds no lock.
+template <typename Enqueue>
+rsort_status run_graphed(const GraphKey& key, void* stream, Enqueue&& enqueue) {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    static uint64_t tick = 0;
+    auto& v = graph_cache();
+    for (auto& g : v) {
+        if (memcmp(&g.key, &key, sizeof(key)) != 0) continue;
+        g.used = ++tick;
+        if (!g.exec) return enqueue((hipStream_t)stream);
+        hipError_t e = hipGraphLaunch(g.exec, (hipStream_t)stream);
+        if (e == hipSuccess) e = hipEventRecord(g.done, (hipStream_t)stream);
+        return e == hipSuccess ? RSORT_SUCCESS : hip_fail(e, "hipGraphLaunch");
+    }
+    const rsort_status st = enqueue((hipStream_t)stream);
+    if (st != RSORT_SUCCESS) return st;
+    GraphEntry ne{};
+    ne.key = key;
+    ne.used = ++tick;
+    const hipStream_t cs = capture_stream(key.dev);
+    if (cs) {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            const rsort_status sc = enqueue(cs);
+            e = hipStreamEndCapture(cs, &g);
+            if (e == hipSuccess && sc != RSORT_SUCCESS) e = hipErrorUnknown;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&ne.exec, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ne.done, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            if (ne.exec) (void)hipGraphExecDestroy(ne.exec);
+            ne.exec = nullptr;
+            (void)hipGetLastError();
+        }
+    }
+    if (v.size() >= kGraphCacheMax) {
+        size_t lru = 0;
+        for (size_t i = 1; i < v.size(); ++i)
+            if (v[i].used < v[lru].used) lru = i;
+        if (v[lru].exec) {
+            (void)hipEventSynchronize(v[lru].done);
+            (void)hipGraphExecDestroy(v[lru].exec);
+            (void)hipEventDestroy(v[lru].done);
+        }
+        v.erase(v.begin() + (ptrdiff_t)lru);
+    }
+    v.push_back(ne);
+    return RSORT_SUCCESS;
+}
+
+GraphKey graph_key(uint32_t entry, const rsort_desc* d, const void* keys, const void* values, const void* workspace,
+                   size_t temp_bytes, uint64_t n, const void* offsets, uint32_t offset_bytes, uint64_t num_segments) {
+    GraphKey k;
+    memset(&k, 0, sizeof(k));
+    (void)hipGetDevice(&k.dev);
+    k.entry = entry;
+    k.desc = *d;
+    k.desc.flags &= ~RSORT_FLAG_GRAPH;
+    k.keys = (uint64_t)(uintptr_t)keys;
+    k.values = (uint64_t)(uintptr_t)values;
+    k.workspace = (uint64_t)(uintptr_t)workspace;
+    k.offsets = (uint64_t)(uintptr_t)offsets;
+    k.n = n;
+    k.num_segments = num_segments;
+    k.temp_bytes = temp_bytes;
+    k.offset_bytes = offset_bytes;
+    return k;
+}
+
 // Control region of a plan: the control header, digit counts, tile tickets,
 // lookback words, digit offsets and histogram partials -- everything from
 // off_ctrl to the end of the layout -- in bytes.
@@ -522,6 +640,12 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
     b.tmp_vals = p.vb_core ? base + p.off_tmp_vals : nullptr;
     b.ctrl_region = base + p.off_ctrl;
     b.index_base = base;
+    if ((d->flags & RSORT_FLAG_GRAPH) && !events) {
+        const GraphKey k = graph_key(0u, d, keys, values, workspace, *temp_bytes, n, nullptr, 0u, 0u);
+        return run_graphed(k, stream, [&](hipStream_t s) {
+            return enqueue_sort(p, keys, values, b, n, s, debug_slots(), nullptr, 0, nullptr);
+        });
+    }
     return enqueue_sort(p, keys, values, b, n, stream, debug_slots(), events, max_events, n_events);
 }
 
@@ -629,7 +753,6 @@ rsort_status run_segmented(const rsort_desc* d, void* keys, void* values, uint64
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
     const int cus = device_cus(dev);
-    hipStream_t s = (hipStream_t)stream;
 
     uint8_t* base = (uint8_t*)(((uintptr_t)workspace + 255u) & ~(uintptr_t)255u);
     SegJob j{};
@@ -662,19 +785,28 @@ rsort_status run_segmented(const rsort_desc* d, void* keys, void* values, uint64
     j.max_events = max_events;
     j.n_events = n_events;
 
-    if (p.index_mode) {
-        e = launch_iota((uint32_t*)j.vals, n, cus, s);
-        if (e != hipSuccess) return hip_fail(e, "index fill");
+    auto enqueue = [&](hipStream_t s) -> rsort_status {
+        hipError_t e2;
+        if (p.index_mode) {
+            e2 = launch_iota((uint32_t*)j.vals, n, cus, s);
+            if (e2 != hipSuccess) return hip_fail(e2, "index fill");
+        }
+        e2 = dispatch_seg(j, s);
+        if (e2 != hipSuccess) return hip_fail(e2, "segmented sort launch");
+        if (p.index_mode) {
+            void* copy = base + p.off_vals_copy;
+            e2 = launch_gather(copy, values, (const uint32_t*)j.vals, n, p.vb_user, cus, s);
+            if (e2 == hipSuccess) e2 = launch_copy(values, copy, n * p.vb_user, cus, s);
+            if (e2 != hipSuccess) return hip_fail(e2, "value gather");
+        }
+        return RSORT_SUCCESS;
+    };
+    if ((d->flags & RSORT_FLAG_GRAPH) && !events) {
+        const GraphKey k = graph_key(1u, d, keys, values, workspace, *temp_bytes, n, offsets, offset_bytes,
+                                     num_segments);
+        return run_graphed(k, stream, enqueue);
     }
-    e = dispatch_seg(j, s);
-    if (e != hipSuccess) return hip_fail(e, "segmented sort launch");
-    if (p.index_mode) {
-        void* copy = base + p.off_vals_copy;
-        e = launch_gather(copy, values, (const uint32_t*)j.vals, n, p.vb_user, cus, s);
-        if (e == hipSuccess) e = launch_copy(values, copy, n * p.vb_user, cus, s);
-        if (e != hipSuccess) return hip_fail(e, "value gather");
-    }
-    return RSORT_SUCCESS;
+    return enqueue((hipStream_t)stream);
 }
 
 }  // namespace

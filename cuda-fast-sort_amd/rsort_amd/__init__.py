@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(PKG_DIR, os.environ.get("RSORT_AMD_LIB", "librsort_amd.s
 
 KEY_UNSIGNED, KEY_SIGNED, KEY_FLOAT = 0, 1, 2
 FLAG_NO_PASS_SKIP = 1
+FLAG_GRAPH = 2  # replay identical calls from a cached hipGraph
 
 SUCCESS = 0
 STATUS_NAMES = {

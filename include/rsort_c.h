@@ -47,6 +47,15 @@ typedef enum rsort_key_kind {
 
 /* Descriptor flags */
 #define RSORT_FLAG_NO_PASS_SKIP 1u /* always run every pass (no uniform-digit skip) */
+/* rsort_sort / rsort_segmented_sort: replay the call from a cached hipGraph.
+ * The first call with a given set of arguments (descriptor, pointers, sizes,
+ * workspace size, device) sorts directly and captures its launch sequence
+ * into an executable graph; later calls with the same arguments launch that
+ * graph on the caller's stream (one launch instead of one per kernel).  The
+ * library keeps 16 such graphs (least recently used one dropped).  The
+ * reference captures and instantiates a graph on every call
+ * (include/rsort/detail/radix_kernel.cuh:583-600). */
+#define RSORT_FLAG_GRAPH 2u
 
 typedef struct rsort_desc {
     uint32_t key_kind;    /* rsort_key_kind */
