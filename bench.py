@@ -1,31 +1,45 @@
 """Benchmark of the onesweep LSD radix sort on one MI355X.
 
-Default workload: sort_pairs of n = 2^28 uint32 keys (uniform over [0, 2^32))
-with uint32 values = original index, ascending -- the key-value configuration
-the headline metric is quoted on.  A step = one complete sort of a fresh copy
-of that input (inputs are resident in HBM before the timed region; each step
-sorts its own copy in place).  Prints one JSON line.
+Default workload (the line's `value`): sort_pairs of n = 2^28 uint32 keys
+(uniform over [0, 2^32)) with uint32 values = original index, ascending.  A
+step = one complete sort of a fresh copy of that input (inputs are resident in
+HBM before the timed region; each step sorts its own copy in place).  The same
+run also measures, each in an object of the line:
+  key_only       the north-star sort: 2^28 u32 keys, with its pass roofline
+  hbm_ceiling    hand-written read / write / copy kernels (the practical ceiling)
+  configs        plain 2^20 u32 (mt19937 seed 0) latency, 2^27 u64 low-16 and
+                 Zipf, 2^27 f32 with +-0/+-inf, 2^26 u128/u64 pairs
+  byte_skip      2^27 u32 from the reference generator, every other byte constant
+  cold_vs_steady 2^27 u32: the process's first sort and cache-flushed sorts
+  segmented      100k lognormal segments (~2^28 u32/u32 keys) in one graph
+                 replay, against one sort call per segment from host C++
+  served         Poisson 5k req/s through the serving loop: p50 / p99 latency
+  cpu_baseline   stable sorts of the same 2^28 records on the host cores
+Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--workload kv|keys] [--log2n 28]
+    python bench.py [--steps K] [--warmup W] [--workload kv|keys|...] [--log2n 28]
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, os.path.join(ROOT, "cuda-fast-sort_amd"))
+PKG = os.path.join(ROOT, "cuda-fast-sort_amd")
+sys.path.insert(0, PKG)
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E peak, GB/s
 METRIC = "GKeys/s + % HBM roofline, 2^28 u32; batched-segment keys/s vs per-call"
 
 
-# Other workload configurations, each a bench line of its own:
-# name -> (description, key kind, key bytes, value bytes, log2 n)
+# Other workload configurations: name -> (description, key kind, key bytes, value bytes, log2 n)
 CONFIG_WORKLOADS = {
     "plain20": ("sort u32 n=2^20, keys = raw mt19937(seed 0) outputs", 0, 4, 0, 20),
     "u64low16": ("sort u64 n=2^27, low 16 bits uniform, upper 48 bits zero", 0, 8, 0, 27),
@@ -42,23 +56,24 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["kv", "keys", "copy", "segments", "serve", *CONFIG_WORKLOADS], default="kv",
-                    help="kv: sort_pairs u32/u32; keys: sort u32; copy: device copy of the keys "
-                         "(stream-copy bandwidth, and the known-size kernel that calibrates HBM counters); "
-                         "segments: one segmented sort of 100k lognormal-sized u32/u32 segments vs one "
-                         "sort_pairs call per segment; serve: Poisson request stream through the serving loop")
+                    help="kv: sort_pairs u32/u32 (the full line); keys: sort u32; copy: the streaming kernels "
+                         "(bandwidth ceiling, and the known-size kernel that calibrates HBM counters); "
+                         "segments: 100k lognormal-sized u32/u32 segments, graph replay vs one call per segment; "
+                         "serve: Poisson request stream through the serving loop")
     ap.add_argument("--segments", type=int, default=100_000)
-    ap.add_argument("--per-call-sample", type=int, default=5000,
-                    help="segments sorted one sort_pairs call each for the per-call comparison")
+    ap.add_argument("--per-call-sample", type=int, default=20_000,
+                    help="segments sorted one rsort_sort call each for the per-call comparison")
     ap.add_argument("--rate", type=float, default=5000.0, help="serve: Poisson arrival rate, requests/s")
-    ap.add_argument("--duration", type=float, default=2.0, help="serve: seconds of arrivals")
+    ap.add_argument("--duration", type=float, default=1.0, help="serve: seconds of arrivals")
     ap.add_argument("--window-us", type=int, default=0,
                     help="serve: batching window (0: launch as soon as a batch slot is free)")
     ap.add_argument("--serve-mode", choices=["direct", "graph", "batched"], default="direct")
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--descending", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="sort with RSORT_FLAG_GRAPH (replayed hipGraph)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the key-only side measurement")
-    ap.add_argument("--cpu-log2n", type=int, default=24, help="CPU baseline sample size")
+    ap.add_argument("--no-extra", action="store_true", help="kv: only the headline sort (no side measurements)")
+    ap.add_argument("--cpu-log2n", type=int, default=28, help="CPU baseline sample size")
     return ap.parse_args()
 
 
@@ -100,36 +115,47 @@ def timed_sorts(rs, desc, ks, vs, ws, tb, n, warmup, steps, profiled):
     return wall_ms, dev_ms, stages
 
 
-def run_sort_bench(rs, n, with_values, descending, steps, warmup, seed=1):
-    """Sorts warmup + steps + profiled fresh copies; returns (wall ms per timed
-    sort, stage ms lists of the profiled sorts, check, keys0, vals0, device ms
-    per timed sort)."""
+def pass_roofline(stages, kb, vb, n, passes):
+    """The pass kernel's roofline from per-stage events: algorithmic bytes per
+    launch 2*n*(kb+vb) over the mean duration of the executed passes."""
+    pass_ms = [sum(s[2 + p] for s in stages) / len(stages) for p in range(passes)]
+    longest = max(pass_ms)
+    executed = [p for p in range(passes) if pass_ms[p] > 0.05 * longest]
+    avg = sum(pass_ms[p] for p in executed) / len(executed)
+    pass_bytes = 2.0 * n * (kb + vb)
+    achieved = pass_bytes / (avg * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "k_onesweep (one executed LSD pass)", "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "bytes_per_launch": pass_bytes, "avg_launch_ms": round(avg, 5)}, executed
+
+
+def run_sort_bench(rs, n, with_values, descending, steps, warmup, seed=1, flags=0):
+    """Sorts warmup + steps + profiled fresh copies of 2^k u32 keys (and
+    index values); returns wall ms per timed sort, device ms per timed sort,
+    the stage ms of the profiled sorts, and the device checks of the last
+    result (order, digit histograms, value pairing)."""
     import torch
 
     dev = torch.device("cuda:0")
     keys0 = gen_u32(n, dev, seed)
     vals0 = torch.arange(n, dtype=torch.int32, device=dev) if with_values else None
     vb = 4 if with_values else 0
-    desc = rs.make_desc(rs.KEY_UNSIGNED, 4, vb, descending)
+    desc = rs.make_desc(rs.KEY_UNSIGNED, 4, vb, descending, flags)
     tb = rs.sort_raw(desc, None, None, None, 0, n)
     ws = torch.empty(tb, dtype=torch.uint8, device=dev)
+    before = rs.digit_counts(keys0, desc, n)
     profiled = min(steps, 5)
     total = warmup + steps + profiled
     ks = [keys0.clone() for _ in range(total)]
     vs = [vals0.clone() for _ in range(total)] if with_values else [None] * total
     torch.cuda.synchronize()
     wall_ms, dev_ms, stages = timed_sorts(rs, desc, ks, vs, ws, tb, n, warmup, steps, profiled)
-    # spot check of the last result (sortedness + pairing) on the device
-    k = ks[-1].view(torch.int32) ^ (-(2**31))  # signed order == unsigned order of the bits
-    ok = bool((k[1:] <= k[:-1]).all().item()) if descending else bool((k[1:] >= k[:-1]).all().item())
+    ok = rs.check_sorted(ks[-1], desc, n) == 0 and torch.equal(rs.digit_counts(ks[-1], desc, n), before)
     if with_values:
-        idx = vs[-1].long()
-        ok = ok and bool((keys0[idx] == ks[-1]).all().item())
-        eq = ks[-1][1:] == ks[-1][:-1]
-        ok = ok and bool((vs[-1][1:][eq] > vs[-1][:-1][eq]).all().item())
+        ok = ok and rs.check_pairings(keys0, ks[-1], vs[-1], 4, 4, n) == 0
     del ks, vs
     torch.cuda.empty_cache()
-    return wall_ms, stages, ok, keys0, vals0, dev_ms
+    return wall_ms, dev_ms, stages, bool(ok), keys0
 
 
 def _fmix64(x):
@@ -152,7 +178,8 @@ def gen_config(name, n, dev, seed):
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
     if name == "plain20":
-        keys = np.random.RandomState(0).randint(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32)
+        # numpy's legacy MT19937 with 32-bit draws gives the raw mt19937 outputs
+        keys = np.random.RandomState(0).randint(0, 2**32, size=n, dtype=np.uint32)
         return torch.from_numpy(keys.view(np.uint8)).to(dev), None
     if name == "u64low16":
         k = torch.randint(0, 1 << 16, (n,), dtype=torch.int64, device=dev, generator=g)
@@ -181,89 +208,220 @@ def gen_config(name, n, dev, seed):
     raise ValueError(name)
 
 
-def bench_config(rs, a):
-    """One of the other workload configurations: GKeys/s, the formula roofline
-    (every pass counted, also those the device skips), per-stage device times,
-    device-side checks (order, digit histograms, value pairing)."""
+def bench_config(rs, name, steps, warmup, descending=False, flags=0):
+    """One workload configuration: GKeys/s, the formula roofline (every pass
+    counted, also those the device skips), the pass roofline, device checks
+    (order, digit histograms, value pairing)."""
     import torch
 
-    desc_s, kind, kb, vb, log2n = CONFIG_WORKLOADS[a.workload]
+    desc_s, kind, kb, vb, log2n = CONFIG_WORKLOADS[name]
     n = 1 << log2n
     dev = torch.device("cuda:0")
-    keys0, vals0 = gen_config(a.workload, n, dev, 21)
-    d = rs.make_desc(kind, kb, vb, a.descending)
+    keys0, vals0 = gen_config(name, n, dev, 21)
+    d = rs.make_desc(kind, kb, vb, descending, flags)
     tb = rs.sort_raw(d, None, None, None, 0, n)
     ws = torch.empty(tb, dtype=torch.uint8, device=dev)
     before = rs.digit_counts(keys0, d, n)
-    profiled = min(a.steps, 5)
-    total = a.warmup + a.steps + profiled
+    profiled = min(steps, 5)
+    total = warmup + steps + profiled
     ks = [keys0.clone() for _ in range(total)]
     vs = [vals0.clone() for _ in range(total)] if vb else [None] * total
     torch.cuda.synchronize()
-    wall_ms, dev_steps, stages = timed_sorts(rs, d, ks, vs, ws, tb, n, a.warmup, a.steps, profiled)
+    wall_ms, dev_steps, stages = timed_sorts(rs, d, ks, vs, ws, tb, n, warmup, steps, profiled)
     ok = rs.check_sorted(ks[-1], d, n) == 0 and torch.equal(rs.digit_counts(ks[-1], d, n), before)
     if vb:
         ok = ok and rs.check_pairings(keys0, ks[-1], vs[-1], kb, vb, n) == 0
     del ks, vs
+    torch.cuda.empty_cache()
     dev_ms = sum(dev_steps) / len(dev_steps)
-    pass_ms = [sum(s[2 + p] for s in stages) / len(stages) for p in range(kb)]
-    longest = max(pass_ms)
-    executed = [p for p in range(kb) if pass_ms[p] > 0.05 * longest]
-    exec_avg = sum(pass_ms[p] for p in executed) / len(executed)
-    pass_bytes = 2.0 * n * (kb + vb)
+    roof, executed = pass_roofline(stages, kb, vb, n, kb)
     formula_bytes = (1 + kb) * 2.0 * n * (kb + vb)
     return {
-        "metric": f"GKeys/s, {desc_s}", "value": round(n / (dev_ms * 1e-3) / 1e9, 3), "unit": "GKeys/s",
-        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dev_ms, 4),
-        "us_per_sort": round(dev_ms * 1e3, 2), "wall_ms_per_step": round(wall_ms, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": {4: "u32", 8: "u64", 16: "u128"}[kb] if kind != 2 else "f32",
-        "data": "synthetic",
-        "config": {"workload": a.workload, "description": desc_s, "n": n, "key_kind": kind, "key_bytes": kb,
-                   "value_bytes": vb, "order": "descending" if a.descending else "ascending"},
-        "roofline": {"bound": "hbm", "kernel": "k_onesweep (one executed LSD pass)",
-                     "achieved": round(pass_bytes / (exec_avg * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(pass_bytes / (exec_avg * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "traffic": None,
-                     "bytes_per_launch": pass_bytes, "avg_launch_ms": round(exec_avg, 5)},
-        "formula_roofline": {"bytes": formula_bytes, "passes_counted": kb, "device_ms": round(dev_ms, 4),
-                             "frac_device": round(formula_bytes / (dev_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
-        "executed_passes": executed,
-        "stages_ms_mean": [round(sum(s[i] for s in stages) / len(stages), 5) for i in range(len(stages[0]))],
-        "checked": bool(ok),
+        "description": desc_s, "n": n, "gkeys_per_s": round(n / (dev_ms * 1e-3) / 1e9, 3),
+        "ms_per_sort": round(dev_ms, 4), "us_per_sort": round(dev_ms * 1e3, 2), "wall_ms_per_sort": round(wall_ms, 4),
+        "roofline": roof,
+        "formula_frac": round(formula_bytes / (dev_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+        "executed_passes": executed, "checked": bool(ok),
     }
 
 
-def measured_traffic(kernel_prefix, n):
-    """HBM bytes per launch of the pass kernel from the committed rocprofv3
-    PMC summary (FETCH_SIZE and WRITE_SIZE, calibrated against a copy of known
-    size), when it was taken on this workload; else None."""
-    path = os.path.join(ROOT, "profiles", "onesweep_kv_u32_2p28_hbm_traffic.md")
+def bench_plain20(rs, steps, warmup):
+    """2^20 u32 keys (raw mt19937 seed 0), microseconds per sort,
+    direct calls and graph replays (RSORT_FLAG_GRAPH), each call timed alone
+    with events around it (the reference's method) and back to back."""
+    import torch
+
+    n = 1 << 20
+    dev = torch.device("cuda:0")
+    keys0, _ = gen_config("plain20", n, dev, 0)
+    out = {"description": CONFIG_WORKLOADS["plain20"][0], "n": n}
+    stream = torch.cuda.current_stream()
+    for label, flags in (("direct", 0), ("graph", rs.FLAG_GRAPH)):
+        d = rs.make_desc(0, 4, 0, False, flags)
+        tb = rs.sort_raw(d, None, None, None, 0, n)
+        ws = torch.empty(tb, dtype=torch.uint8, device=dev)
+        k = keys0.clone()
+        for _ in range(warmup + 2):
+            k.copy_(keys0)
+            rs.sort_raw(d, k.data_ptr(), None, ws.data_ptr(), tb, n, stream)
+        torch.cuda.synchronize()
+        alone = []
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for _ in range(max(steps, 10)):
+            k.copy_(keys0)
+            torch.cuda.synchronize()
+            e0.record(stream)
+            rs.sort_raw(d, k.data_ptr(), None, ws.data_ptr(), tb, n, stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            alone.append(e0.elapsed_time(e1) * 1e3)
+        ok = rs.check_sorted(k, d, n) == 0 and torch.equal(rs.digit_counts(k, d, n), rs.digit_counts(keys0, d, n))
+        # back to back: K sorts of separate copies between two events
+        copies = [keys0.clone() for _ in range(max(steps, 10))]
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for c in copies:
+            rs.sort_raw(d, c.data_ptr(), None, ws.data_ptr(), tb, n, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        b2b = e0.elapsed_time(e1) * 1e3 / len(copies)
+        alone.sort()
+        out[label] = {"us_per_sort_alone_median": round(alone[len(alone) // 2], 2),
+                      "us_per_sort_back_to_back": round(b2b, 2), "checked": bool(ok)}
+    out["us_per_sort"] = out["direct"]["us_per_sort_alone_median"]
+    return out
+
+
+def bench_init_keys_mode(rs, mode, n, steps, warmup):
+    """u32 keys from the reference generator (mode 1 random, 2 byte_skip),
+    steady state: GKeys/s and the passes the device skipped."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    keys0 = torch.empty(n, dtype=torch.int32, device=dev)
+    rs.init_keys(keys0, 4, mode, rs.set_seed_radix(0), n)
+    d = rs.make_desc(0, 4, 0)
+    tb = rs.sort_raw(d, None, None, None, 0, n)
+    ws = torch.empty(tb, dtype=torch.uint8, device=dev)
+    before = rs.digit_counts(keys0, d, n)
+    profiled = 2
+    ks = [keys0.clone() for _ in range(warmup + steps + profiled)]
+    torch.cuda.synchronize()
+    wall_ms, dev_steps, stages = timed_sorts(rs, d, ks, [None] * len(ks), ws, tb, n, warmup, steps, profiled)
+    ok = rs.check_sorted(ks[-1], d, n) == 0 and torch.equal(rs.digit_counts(ks[-1], d, n), before)
+    ms = sum(dev_steps) / len(dev_steps)
+    _, executed = pass_roofline(stages, 4, 0, n, 4)
+    del ks
+    torch.cuda.empty_cache()
+    return {"gkeys_per_s": round(n / (ms * 1e-3) / 1e9, 3), "ms_per_sort": round(ms, 4),
+            "executed_passes": executed, "checked": bool(ok)}
+
+
+def bench_cold(rs):
+    """The process's first sort (no warm-up of any kind): 2^27 u32 keys from
+    the reference generator, timed with events around the call (device) and
+    the host clock (first-use costs included); then five sorts each after the
+    caches are flushed (a 1 GiB write) and the GPU idles for 50 ms."""
+    import torch
+
+    n = 1 << 27
+    dev = torch.device("cuda:0")
+    keys = torch.empty(n, dtype=torch.int32, device=dev)
+    rs.init_keys(keys, 4, 1, rs.set_seed_radix(0), n)
+    keys0 = keys.clone()
+    d = rs.make_desc(0, 4, 0)
+    tb = rs.sort_raw(d, None, None, None, 0, n)
+    ws = torch.empty(tb, dtype=torch.uint8, device=dev)
+    flush = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    rs.sort_raw(d, keys.data_ptr(), None, ws.data_ptr(), tb, n, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    first_wall = (time.perf_counter() - t0) * 1e3
+    first_dev = e0.elapsed_time(e1)
+    ok = rs.check_sorted(keys, d, n) == 0
+    flushed = []
+    for _ in range(5):
+        keys.copy_(keys0)
+        flush.fill_(1)
+        torch.cuda.synchronize()
+        time.sleep(0.05)
+        e0.record(stream)
+        rs.sort_raw(d, keys.data_ptr(), None, ws.data_ptr(), tb, n, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        flushed.append(e0.elapsed_time(e1))
+    del flush
+    torch.cuda.empty_cache()
+    fl = sum(flushed) / len(flushed)
+    return {"n": n, "first_call_device_ms": round(first_dev, 4), "first_call_wall_ms": round(first_wall, 3),
+            "first_call_gkeys_per_s": round(n / (first_dev * 1e-3) / 1e9, 3),
+            "cache_flushed_ms": round(fl, 4), "cache_flushed_gkeys_per_s": round(n / (fl * 1e-3) / 1e9, 3),
+            "checked": bool(ok)}
+
+
+def stream_ceiling(rs, nbytes=1 << 31, reps=5):
+    """Hand-written streaming kernels (rsort_stream_bandwidth): GB/s of a read,
+    a write and a copy of nbytes, best over workgroups per CU in {4, 8, 16}."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    src = torch.ones(nbytes, dtype=torch.uint8, device=dev)
+    dst = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    sink = torch.empty(4096, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+    out = {"bytes": nbytes}
+    for name, mode, moved in (("read", 0, nbytes), ("write", 1, nbytes), ("copy", 2, 2 * nbytes)):
+        best = (0.0, 0)
+        for bpc in (4, 8, 16):
+            dptr = sink.data_ptr() if mode == 0 else dst.data_ptr()
+            rs.stream_bandwidth(mode, dptr, src.data_ptr(), nbytes, bpc, stream)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                rs.stream_bandwidth(mode, dptr, src.data_ptr(), nbytes, bpc, stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            gbs = moved * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+            if gbs > best[0]:
+                best = (gbs, bpc)
+        out[name + "_gbs"] = round(best[0], 1)
+        out[name + "_blocks_per_cu"] = best[1]
+    del src, dst
+    torch.cuda.empty_cache()
+    return out
+
+
+def kernel_sources_sha256():
+    """Hash of the library sources (every file under csrc/): a traffic figure
+    measured on other sources is not reported."""
+    h = hashlib.sha256()
+    csrc = os.path.join(PKG, "csrc")
+    for f in sorted(os.listdir(csrc)):
+        with open(os.path.join(csrc, f), "rb") as fh:
+            h.update(f.encode() + b"\0" + fh.read())
+    return h.hexdigest()
+
+
+def measured_traffic(profile, kernel_prefix, n):
+    """HBM bytes per launch of the pass kernel from a committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE, calibrated against a copy of known
+    size), when it was taken on this workload and on these very sources."""
+    path = os.path.join(ROOT, "profiles", profile)
     try:
         text = open(path).read()
         block = text[text.index("```json") + 7:]
         d = json.loads(block[:block.index("```")])
     except (OSError, ValueError):
         return None
-    if d.get("n") == n and d.get("kernel", "").startswith(kernel_prefix):
+    if d.get("n") == n and d.get("kernel", "").startswith(kernel_prefix) and \
+            d.get("source_sha256") == kernel_sources_sha256():
         return d["traffic_bytes"]
     return None
-
-
-def copy_bandwidth(n, steps):
-    """Device-to-device copy of n u32 (read n*4 + write n*4 bytes); returns GB/s."""
-    import torch
-
-    src = gen_u32(n, torch.device("cuda:0"), 3)
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(steps):
-        dst.copy_(src)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / steps
-    return 2.0 * n * 4 / (ms * 1e-3) / 1e9, ms
 
 
 def lognormal_sizes(count, seed=5, median=2048.0, sigma=0.7356, lo=32, hi=1 << 22):
@@ -276,8 +434,11 @@ def lognormal_sizes(count, seed=5, median=2048.0, sigma=0.7356, lo=32, hi=1 << 2
     return np.clip(v, lo, hi).astype(np.int64)
 
 
-def bench_segments(rs, a):
-    """One segmented sort of the whole batch per step vs one sort_pairs call per segment."""
+def bench_segments(rs, a, steps, warmup):
+    """configs[4] batch: one segmented sort of 100k lognormal segments replayed
+    from its graph (device time per replay, device checks), and from host C++
+    (cli/seg_percall) the same batch's graph replay against one rsort_sort
+    call per segment over the first per_call_sample segments."""
     import numpy as np
     import torch
 
@@ -288,85 +449,88 @@ def bench_segments(rs, a):
     offsets = torch.from_numpy(offs_np).to(dev)
     keys0 = gen_u32(n, dev, 7)
     vals0 = torch.arange(n, dtype=torch.int32, device=dev)
-    desc = rs.make_desc(rs.KEY_UNSIGNED, 4, 4, a.descending)
+    desc = rs.make_desc(rs.KEY_UNSIGNED, 4, 4, False, rs.FLAG_GRAPH)
     tb = rs.segmented_sort_raw(desc, None, None, n, None, 8, a.segments, None, 0)
     ws = torch.empty(tb, dtype=torch.uint8, device=dev)
-    total = a.steps + a.warmup
-    ks = [keys0.clone() for _ in range(total)]
-    vs = [vals0.clone() for _ in range(total)]
-    torch.cuda.synchronize()
+    k, v = keys0.clone(), vals0.clone()
     stream = torch.cuda.current_stream()
-    call = lambda i: rs.segmented_sort_raw(desc, ks[i].data_ptr(), vs[i].data_ptr(), n, offsets.data_ptr(), 8,
-                                           a.segments, ws.data_ptr(), tb, stream)
-    for i in range(a.warmup):
-        call(i)
+    call = lambda: rs.segmented_sort_raw(desc, k.data_ptr(), v.data_ptr(), n, offsets.data_ptr(), 8, a.segments,
+                                         ws.data_ptr(), tb, stream)
+    for _ in range(warmup):
+        k.copy_(keys0)
+        v.copy_(vals0)
+        call()
     torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps + 1)]
-    t0 = time.perf_counter()
-    ev[0].record(stream)
-    for j, i in enumerate(range(a.warmup, total)):
-        call(i)
-        ev[j + 1].record(stream)
-    torch.cuda.synchronize()
-    wall_ms = (time.perf_counter() - t0) * 1e3 / a.steps
-    dev_ms = [ev[j].elapsed_time(ev[j + 1]) for j in range(a.steps)]
-    ms = sum(dev_ms) / len(dev_ms)
-    # every segment sorted, and values still pair with their keys
-    k = ks[-1].view(torch.int32) ^ (-(2**31))
-    up = k[1:] >= k[:-1]
+    ms = []
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(steps):
+        k.copy_(keys0)
+        v.copy_(vals0)
+        e0.record(stream)
+        call()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    dev_ms = sum(ms) / len(ms)
+    # every segment sorted, values point at their keys and stay in their segment
+    ku = k ^ (-(2**31))
     seg_start = torch.zeros(n, dtype=torch.bool, device=dev)
     seg_start[offsets[1:-1]] = True
-    ok = bool((up | seg_start[1:]).all().item()) and bool((keys0[vs[-1].long()] == ks[-1]).all().item())
+    ok = bool(((ku[1:] >= ku[:-1]) | seg_start[1:]).all().item()) and \
+        rs.check_pairings(keys0, k, v, 4, 4, n) == 0
+    seg_of = torch.repeat_interleave(torch.arange(a.segments, dtype=torch.int32, device=dev),
+                                     torch.from_numpy(sizes).to(dev))
+    ok = ok and bool((seg_of[v.long()] == seg_of).all().item())
     lds_cap = rs.segmented_plan(desc, n, 8, a.segments).bin_capacity[4]
-    small = sizes[sizes <= lds_cap].sum()
-    big = sizes[sizes > lds_cap].sum()
-    kvb = 8
-    bytes_ = 2.0 * small * kvb + 5 * 2.0 * big * kvb  # LDS segments: one read + one write; long: formula
-    batched_k, batched_v = ks[-1], vs[-1]
-    del ks, vs
-    # per-call comparison: the same segments, one sort_pairs call each
-    m = min(a.per_call_sample, a.segments)
-    pk, pv = keys0.clone(), vals0.clone()
-    fdesc = rs.make_desc(rs.KEY_UNSIGNED, 4, 4, a.descending)
-    ftb = rs.sort_raw(fdesc, None, None, None, 0, int(sizes[:m].max()))
-    fws = torch.empty(ftb, dtype=torch.uint8, device=dev)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    kp, vp = pk.data_ptr(), pv.data_ptr()
-    for i in range(m):
-        o, c = int(offs_np[i]), int(sizes[i])
-        rs.sort_raw(fdesc, kp + 4 * o, vp + 4 * o, fws.data_ptr(), ftb, c, stream)
-    torch.cuda.synchronize()
-    per_call_s = time.perf_counter() - t0
-    per_call_keys = int(sizes[:m].sum())
-    same = bool((pk[:per_call_keys] == batched_k[:per_call_keys]).all().item()) and bool(
-        (pv[:per_call_keys] == batched_v[:per_call_keys]).all().item())
-    batched = n / (ms * 1e-3) / 1e9
-    per_call = per_call_keys / per_call_s / 1e9
-    return {
-        "metric": "segmented sort keys/s, 100k lognormal segments vs per-call sort_pairs",
-        "value": round(batched, 3), "unit": "GKeys/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": round(ms, 4), "wall_ms_per_step": round(wall_ms, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-        "config": {"workload": f"segmented sort_pairs u32/u32, {a.segments} segments, lognormal sizes median 2048 "
-                               f"sigma 0.7356 clipped [32, 2^22], n={n}", "segments": a.segments, "n": n,
-                   "lds_capacity": int(lds_cap), "keys_in_lds_segments": int(small), "keys_in_long_segments": int(big)},
+    small = int(sizes[sizes <= lds_cap].sum())
+    big = n - small
+    bytes_ = 2.0 * small * 8 + 5 * 2.0 * big * 8  # LDS segments: one read + one write; long: the formula
+    del k, v, ku, seg_start, seg_of, ws
+    torch.cuda.empty_cache()
+    batched = n / (dev_ms * 1e-3) / 1e9
+    res = {
+        "description": f"{a.segments} segments of u32/u32, lognormal sizes median 2048 sigma 0.7356 clipped "
+                       f"[32, 2^22], keys uniform; one segmented sort replayed from its hipGraph",
+        "n": n, "segments": a.segments, "gkeys_per_s": round(batched, 3), "ms_per_batch": round(dev_ms, 4),
+        "keys_in_lds_segments": small, "keys_in_long_segments": big,
         "roofline": {"bound": "hbm", "kernel": "whole segmented sort (device time)",
-                     "achieved": round(bytes_ / (ms * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(bytes_ / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "traffic": None,
-                     "bytes": bytes_},
-        "per_call": {"gkeys_per_s": round(per_call, 5), "calls": m, "keys": per_call_keys,
-                     "us_per_call": round(per_call_s * 1e6 / m, 2),
-                     "note": "one rsort_sort call (sort_pairs) per segment from Python via ctypes, same stream"},
-        "batched_over_per_call": round(batched / per_call, 1),
+                     "achieved": round(bytes_ / (dev_ms * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(bytes_ / (dev_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "bytes": bytes_},
         "checked": ok,
-        "per_call_equals_batched": same,
     }
+    exe = os.path.join(PKG, "cli", "seg_percall")
+    with tempfile.NamedTemporaryFile(suffix=".bin", delete=False) as f:
+        f.write(sizes.astype("<u8").tobytes())
+        path = f.name
+    try:
+        r = subprocess.run([exe, path, "--calls", str(min(a.per_call_sample, a.segments)), "--replays", "10"],
+                           capture_output=True, text=True, timeout=300)
+        cpp = json.loads(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else {"error": r.stderr[-500:]}
+    finally:
+        os.unlink(path)
+    if "per_call" in cpp:
+        res["per_call"] = dict(cpp["per_call"], note="one rsort_sort (sort_pairs u32/u32) per segment from host C++ "
+                                                     "over the C ABI, back to back on one stream")
+        res["cpp_batched"] = cpp["batched"]
+        res["batched_over_per_call"] = cpp["batched_over_per_call"]
+        res["checked"] = res["checked"] and cpp["batched"]["violations"] == 0 and cpp["per_call"]["violations"] == 0
+    else:
+        res["per_call"] = cpp
+    return res
+
+
+def _submit_raw(rs, srv, kptr, vptr, n):
+    import ctypes
+
+    rid = ctypes.c_uint64(0)
+    rs._check(rs.lib().rsort_serve_submit(srv._h, kptr, vptr, n, ctypes.byref(rid)))
+    return int(rid.value)
 
 
 def bench_serve(rs, a):
-    """Poisson request stream through the serving loop: latency percentiles and
-    throughput; then the same requests submitted at once (saturation)."""
+    """configs[4] stream: Poisson arrivals through the serving loop, latency
+    percentiles and throughput; then the same requests submitted at once
+    (saturation)."""
     import numpy as np
     import torch
 
@@ -382,15 +546,12 @@ def bench_serve(rs, a):
     torch.cuda.synchronize()
     srv = rs.Server(rs.KeyType(rs.KEY_UNSIGNED, 4), value_bytes=4, batch_keys=1 << 24, batch_requests=8192,
                     window_us=a.window_us, depth=3, mode=a.serve_mode)
-    # warm-up requests (not timed)
     wk, wv = keys_ref.clone(), vals.clone()
     warm = [_submit_raw(rs, srv, wk.data_ptr() + 4 * int(offs[i]), wv.data_ptr() + 4 * int(offs[i]), int(sizes[i]))
             for i in range(min(50, count))]
     for w in warm:
         srv.wait(w)
     del wk, wv
-    keys.copy_(keys_ref)
-    vals.copy_(torch.arange(n, dtype=torch.int32, device=dev))
     torch.cuda.synchronize()
     st0 = srv.stats()
     rng = np.random.default_rng(17)
@@ -409,96 +570,82 @@ def bench_serve(rs, a):
     lat = np.array([x.latency_us for x in infos], dtype=np.float64)
     q = np.array([x.queue_us for x in infos], dtype=np.float64)
     st = srv.stats()
-    # correctness: every request sorted with its values
     k = keys.view(torch.int32) ^ (-(2**31))
     starts = torch.zeros(n, dtype=torch.bool, device=dev)
     starts[torch.from_numpy(offs[1:-1]).to(dev)] = True
-    ok = bool(((k[1:] >= k[:-1]) | starts[1:]).all().item()) and bool((keys_ref[vals.long()] == keys).all().item())
-    # saturation: the same requests all at once (one submit_many call)
+    ok = bool(((k[1:] >= k[:-1]) | starts[1:]).all().item()) and rs.check_pairings(keys_ref, keys, vals, 4, 4, n) == 0
     keys.copy_(keys_ref)
+    vals.copy_(torch.arange(n, dtype=torch.int32, device=dev))
     kptrs = [kp + 4 * int(offs[i]) for i in range(count)]
     vptrs = [vp + 4 * int(offs[i]) for i in range(count)]
-    ns = [int(x) for x in sizes]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ids2 = srv.submit_many(kptrs, vptrs, ns)
+    ids2 = srv.submit_many(kptrs, vptrs, [int(x) for x in sizes])
     for i in ids2:
         srv.wait(i)
     t_sat = time.perf_counter() - t0
     st2 = srv.stats()
     srv.close()
     return {
-        "metric": "serving loop latency p50/p99 at a Poisson request rate, lognormal request sizes",
-        "value": round(float(np.percentile(lat, 99)), 1), "unit": "us (p99 latency)", "n_gpus": 1,
-        "higher_is_better": False, "dtype": "u32", "data": "synthetic",
-        "config": {"workload": f"sort_pairs u32/u32 requests, Poisson {a.rate:g} req/s for {a.duration:g} s, sizes "
-                               "lognormal median 2048 sigma 0.7356 clipped [32, 2^22]", "requests": count,
-                   "keys": n, "batch_keys": 1 << 24, "batch_requests": 8192, "window_us": a.window_us, "depth": 3,
-                   "mode": a.serve_mode},
+        "description": f"sort_pairs u32/u32 requests, Poisson {a.rate:g} req/s for {a.duration:g} s, sizes lognormal "
+                       "median 2048 sigma 0.7356 clipped [32, 2^22]",
+        "requests": count, "keys": n, "mode": a.serve_mode, "depth": 3, "window_us": a.window_us,
         "latency_us": {"p50": round(float(np.percentile(lat, 50)), 1), "p90": round(float(np.percentile(lat, 90)), 1),
                        "p99": round(float(np.percentile(lat, 99)), 1), "max": round(float(lat.max()), 1),
-                       "mean": round(float(lat.mean()), 1), "queue_p50": round(float(np.percentile(q, 50)), 1)},
-        "offered_keys_per_s": round(n / a.duration / 1e9, 5),
-        "achieved_keys_per_s": round(n / (t_end - t_start) / 1e9, 5),
+                       "queue_p50": round(float(np.percentile(q, 50)), 1)},
+        "offered_gkeys_per_s": round(n / a.duration / 1e9, 5),
+        "achieved_gkeys_per_s": round(n / (t_end - t_start) / 1e9, 5),
         "batches": int(st.batches - st0.batches),
-        "mean_requests_per_batch": round(count / max(1, int(st.batches - st0.batches)), 2),
-        "saturation": {"gkeys_per_s": round(n / t_sat / 1e9, 3), "seconds": round(t_sat, 4),
-                       "batches": int(st2.batches - st.batches)},
+        "saturation_gkeys_per_s": round(n / t_sat / 1e9, 3), "saturation_batches": int(st2.batches - st.batches),
         "checked": ok,
     }
 
 
-def _submit_raw(rs, srv, kptr, vptr, n):
-    import ctypes
+def cpu_baseline(keys0, with_values, log2n):
+    """Stable sorts of the same (key, index) records on the host cores:
+    std::stable_sort on one core, and the oracle's multi-threaded stable sort
+    (sorted runs merged in parallel) on every core; std::sort(par_unseq) too."""
+    import numpy as np
 
-    rid = ctypes.c_uint64(0)
-    rs._check(rs.lib().rsort_serve_submit(srv._h, kptr, vptr, n, ctypes.byref(rid)))
-    return int(rid.value)
+    from oracle import oracle
 
-
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
-                          "n_gpus": a.gpus}))
-        sys.exit(2)
-    import torch
-    import rsort_amd as rs
-
-    rs.lib()
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    n = 1 << a.log2n
-    if a.workload == "segments":
-        print(json.dumps(bench_segments(rs, a)), flush=True)
-        return
-    if a.workload == "serve":
-        print(json.dumps(bench_serve(rs, a)), flush=True)
-        return
-    if a.workload in CONFIG_WORKLOADS:
-        print(json.dumps(bench_config(rs, a)), flush=True)
-        return
-    if a.workload == "copy":
-        gbs, ms = copy_bandwidth(n, a.steps)
-        print(json.dumps({"metric": "device copy bandwidth", "value": round(gbs, 1), "unit": "GB/s",
-                          "ms_per_step": round(ms, 4), "bytes_per_step": 8 * n, "n_gpus": 1,
-                          "steps": a.steps, "higher_is_better": True}), flush=True)
-        return
-    kv = a.workload == "kv"
-    kb, vb = 4, (4 if kv else 0)
-    passes = 4
-    ms, stages, ok, keys0, _, dev_steps = run_sort_bench(rs, n, kv, a.descending, a.steps, a.warmup)
-    # stage layout (profiled sorts): [clear, histogram, pass0..pass3] (+ copy-back when present)
-    pass_ms = [s[2 + p] for s in stages for p in range(passes)]
-    hist_ms = [s[1] for s in stages]
-    sort_dev_ms = sum(dev_steps) / len(dev_steps)
-    avg_pass = sum(pass_ms) / len(pass_ms)
-    pass_bytes = 2.0 * n * (kb + vb)  # read + write of keys and values per pass
-    achieved = pass_bytes / (avg_pass * 1e-3) / 1e9
-    formula_bytes = (1 + passes) * 2.0 * n * (kb + vb)
-    gkeys = n / (ms * 1e-3) / 1e9
+    m = 1 << log2n
+    sample_k = keys0[:m].cpu().numpy().view("uint32")
+    sample_v = np.arange(m, dtype=np.uint32) if with_values else None
+    cores = os.cpu_count() or 1
+    t_par = oracle.time_stable_sort_par_u32(sample_k, sample_v, cores, 3)
+    t_one = oracle.time_stable_sort_u32(sample_k, sample_v, 1)
+    what = "(u32 key, u32 value) records" if with_values else "u32 keys"
     out = {
-        "metric": METRIC,
-        "value": round(gkeys, 3),
+        "value": round(m / t_par / 1e9, 5), "unit": "GKeys/s", "cores": cores, "kind": "port",
+        "sample": f"multi-threaded stable sort (runs std::stable_sort-ed and merged in parallel) of 2^{log2n} {what}, "
+                  "the same input as the GPU sort, median of 3 runs",
+        "single_core": {"value": round(m / t_one / 1e9, 5), "unit": "GKeys/s", "cores": 1,
+                        "sample": f"std::stable_sort of the same 2^{log2n} {what}, one run"},
+        "host_cores_available": cores,
+    }
+    if oracle.pstl_parallel_backend():
+        t_pu = oracle.time_sort_par_u32(sample_k, sample_v, 1)
+        out["par_unseq"] = {"value": round(m / t_pu / 1e9, 5), "unit": "GKeys/s", "cores": cores,
+                            "note": "std::sort(std::execution::par_unseq), not stable"}
+    else:
+        out["par_unseq"] = {"note": "libstdc++ without TBB runs std::sort(par_unseq) serially: not timed"}
+    return out
+
+
+def headline(rs, a, n, kv):
+    kb, vb, passes = 4, (4 if kv else 0), 4
+    flags = rs.FLAG_GRAPH if a.graph else 0
+    ms, dev_steps, stages, ok, keys0 = run_sort_bench(rs, n, kv, a.descending, a.steps, a.warmup, flags=flags)
+    sort_dev_ms = sum(dev_steps) / len(dev_steps)
+    roof, _ = pass_roofline(stages, kb, vb, n, passes)
+    roof["traffic"] = measured_traffic("onesweep_kv_u32_2p28_hbm_traffic.md" if kv else
+                                       "onesweep_keys_u32_2p28_hbm_traffic.md",
+                                       f"k_onesweep<4,{vb},", n)
+    formula_bytes = (1 + passes) * 2.0 * n * (kb + vb)
+    out = {
+        "metric": METRIC if kv else "GKeys/s + % HBM roofline, 2^28 u32 key-only",
+        "value": round(n / (ms * 1e-3) / 1e9, 3),
         "unit": "GKeys/s",
         "n_gpus": 1,
         "steps": a.steps,
@@ -516,55 +663,87 @@ def main():
             "n": n, "key": "u32", "value": "u32" if kv else None,
             "order": "descending" if a.descending else "ascending",
         },
-        "roofline": {
-            "bound": "hbm", "kernel": "k_onesweep (one LSD pass)",
-            "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": measured_traffic("k_onesweep<4,4," if kv else "k_onesweep<4,0,", n),
-            "bytes_per_launch": pass_bytes, "avg_launch_ms": round(avg_pass, 5),
-        },
+        "roofline": roof,
         "formula_roofline": {
             "bytes": formula_bytes, "device_ms": round(sort_dev_ms, 4),
             "frac_device": round(formula_bytes / (sort_dev_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
             "frac_wall": round(formula_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
         },
         "stages_ms_mean": [round(sum(s[i] for s in stages) / len(stages), 5) for i in range(len(stages[0]))],
-        "histogram_ms": round(sum(hist_ms) / len(hist_ms), 5),
+        "histogram_ms": round(sum(s[1] for s in stages) / len(stages), 5),
         "checked": ok,
     }
-    cgbs, _ = copy_bandwidth(n, 5)
-    out["stream_copy_gbs"] = round(cgbs, 1)  # measured device copy bandwidth on this box
-    if kv and not a.no_extra:
-        kms, kst, kok, _, _, kdevs = run_sort_bench(rs, n, False, a.descending, max(3, a.steps // 2), 2, seed=2)
+    return out, keys0
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import torch
+
+    import rsort_amd as rs
+
+    rs.lib()
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    n = 1 << a.log2n
+    if a.workload == "segments":
+        print(json.dumps(bench_segments(rs, a, a.steps, a.warmup)), flush=True)
+        return
+    if a.workload == "serve":
+        print(json.dumps(bench_serve(rs, a)), flush=True)
+        return
+    if a.workload in CONFIG_WORKLOADS:
+        r = bench_plain20(rs, a.steps, a.warmup) if a.workload == "plain20" else \
+            bench_config(rs, a.workload, a.steps, a.warmup, a.descending, rs.FLAG_GRAPH if a.graph else 0)
+        print(json.dumps(r), flush=True)
+        return
+    if a.workload == "copy":
+        print(json.dumps({"metric": "streaming kernels: read / write / copy bandwidth", "unit": "GB/s",
+                          **stream_ceiling(rs, n * 4, a.steps)}), flush=True)
+        return
+    kv = a.workload == "kv"
+    extra = kv and not a.no_extra
+    cold = bench_cold(rs) if extra else None  # first: the process's first sort
+    out, keys0 = headline(rs, a, n, kv)
+    if extra:
+        print("[bench] headline done", file=sys.stderr, flush=True)
+        kms, kdevs, kst, kok, _ = run_sort_bench(rs, n, False, a.descending, max(3, a.steps // 2), 2, seed=2)
         kdev = sum(kdevs) / len(kdevs)
-        kbytes = (1 + passes) * 2.0 * n * kb
+        kroof, _ = pass_roofline(kst, 4, 0, n, 4)
+        kroof["traffic"] = measured_traffic("onesweep_keys_u32_2p28_hbm_traffic.md", "k_onesweep<4,0,", n)
+        kbytes = 5 * 2.0 * n * 4
         out["key_only"] = {
             "gkeys_per_s": round(n / (kms * 1e-3) / 1e9, 3), "ms_per_step": round(kms, 4),
             "device_ms": round(kdev, 4), "formula_frac_device": round(kbytes / (kdev * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
-            "formula_frac_wall": round(kbytes / (kms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "checked": kok,
+            "formula_frac_wall": round(kbytes / (kms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "roofline": kroof,
+            "stages_ms_mean": [round(sum(s[i] for s in kst) / len(kst), 5) for i in range(len(kst[0]))],
+            "checked": kok,
         }
+        print("[bench] key-only done", file=sys.stderr, flush=True)
+        out["hbm_ceiling"] = stream_ceiling(rs)
+        cp = out["hbm_ceiling"]["copy_gbs"]
+        out["roofline"]["frac_of_copy_ceiling"] = round(out["roofline"]["achieved"] / cp, 4)
+        out["key_only"]["roofline"]["frac_of_copy_ceiling"] = round(kroof["achieved"] / cp, 4)
+        print("[bench] ceiling done", file=sys.stderr, flush=True)
+        out["configs"] = {"plain20": bench_plain20(rs, a.steps, a.warmup)}
+        for name in ("u64low16", "zipf", "f32", "u128kv"):
+            out["configs"][name] = bench_config(rs, name, max(3, a.steps // 2), 2)
+            print(f"[bench] {name} done", file=sys.stderr, flush=True)
+        out["plain20_us"] = out["configs"]["plain20"]["us_per_sort"]
+        rnd = bench_init_keys_mode(rs, 1, 1 << 27, max(3, a.steps // 2), 2)
+        bsk = bench_init_keys_mode(rs, 2, 1 << 27, max(3, a.steps // 2), 2)
+        out["byte_skip"] = {"description": "u32 n=2^27 from the reference generator: random vs byte_skip (every "
+                                           "other byte constant), steady state", "random": rnd, "byte_skip": bsk}
+        out["cold_vs_steady"] = dict(cold, steady_gkeys_per_s=rnd["gkeys_per_s"])
+        out["segmented"] = bench_segments(rs, a, max(3, a.steps // 2), 2)
+        print("[bench] segmented done", file=sys.stderr, flush=True)
+        out["served"] = bench_serve(rs, a)
+        print("[bench] served done", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
-        from oracle import oracle
-
-        m = 1 << a.cpu_log2n
-        sample_k = keys0[:m].cpu().numpy().view("uint32")
-        sample_v = None
-        if kv:
-            import numpy as np
-
-            sample_v = np.arange(m, dtype=np.uint32)
-        t_st = oracle.time_stable_sort_u32(sample_k, sample_v, 3)
-        t_par = oracle.time_sort_par_u32(sample_k, sample_v, 3)
-        par_threads = os.cpu_count() if oracle.pstl_parallel_backend() else 1
-        out["cpu_baseline"] = {
-            "value": round(m / t_st / 1e9, 5), "unit": "GKeys/s", "cores": 1, "kind": "port",
-            "sample": f"std::stable_sort of 2^{a.cpu_log2n} (u32 key{', u32 value' if kv else ''}) records "
-                      f"from the same input, median of 3 runs",
-            "par_unseq": {"value": round(m / t_par / 1e9, 5), "unit": "GKeys/s", "cores": par_threads,
-                          "note": "std::sort(std::execution::par_unseq); libstdc++ without TBB runs it serially"
-                          if par_threads == 1 else "std::sort(std::execution::par_unseq)"},
-            "host_cores_available": os.cpu_count(),
-        }
+        out["cpu_baseline"] = cpu_baseline(keys0, kv, min(a.cpu_log2n, a.log2n))
     print(json.dumps(out), flush=True)
 
 

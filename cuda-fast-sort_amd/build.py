@@ -77,19 +77,25 @@ def build(jobs: int | None = None, verbose: bool = True, variant: str = "release
     return lib
 
 
+CLI_PROGRAMS = ("radix_gpu", "seg_percall")
+
+
 def build_cli(verbose: bool = True) -> str:
-    """cli/radix_gpu: the benchmark / validation driver, host C++ over the C ABI."""
-    exe = os.path.join(HERE, "cli", "radix_gpu")
-    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-           os.path.join(HERE, "cli", "radix_gpu.cpp"), "-o", exe + ".tmp", "-L" + HERE, "-lrsort_amd",
-           "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath,$ORIGIN/.."]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"radix_gpu build failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(exe + ".tmp", exe)
-    if verbose:
-        print(f"[build] {exe}", flush=True)
-    return exe
+    """Host C++ programs over the C ABI: cli/radix_gpu (the benchmark /
+    validation driver) and cli/seg_percall (batched segmented sort against one
+    sort call per segment)."""
+    for name in CLI_PROGRAMS:
+        exe = os.path.join(HERE, "cli", name)
+        cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+               os.path.join(HERE, "cli", name + ".cpp"), "-o", exe + ".tmp", "-L" + HERE, "-lrsort_amd",
+               "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name} build failed:\n{r.stdout}\n{r.stderr}")
+        os.replace(exe + ".tmp", exe)
+        if verbose:
+            print(f"[build] {exe}", flush=True)
+    return os.path.join(HERE, "cli", CLI_PROGRAMS[0])
 
 
 if __name__ == "__main__":

@@ -128,6 +128,8 @@ def lib() -> ctypes.CDLL:
         L.rsort_sort.restype = i32
         L.rsort_sort_noalloc.argtypes = [ctypes.POINTER(Desc), vp, vp, vp, vp, u64, vp]
         L.rsort_sort_noalloc.restype = i32
+        L.rsort_stream_bandwidth.argtypes = [ctypes.c_uint32, vp, vp, u64, ctypes.c_uint32, vp]
+        L.rsort_stream_bandwidth.restype = i32
         L.rsort_sort_profiled.argtypes = [ctypes.POINTER(Desc), vp, vp, vp, ctypes.POINTER(ctypes.c_size_t), u64, vp,
                                           ctypes.POINTER(ctypes.c_float), i32, ctypes.POINTER(i32)]
         L.rsort_sort_profiled.restype = i32
@@ -239,6 +241,12 @@ def sort_noalloc_raw(desc: Desc, keys_ptr: int | None, vals_ptr: int | None, key
     state; allocates nothing."""
     _check(lib().rsort_sort_noalloc(ctypes.byref(desc), keys_ptr, vals_ptr, keys_alt_ptr, vals_alt_ptr, n,
                                     _stream_ptr(stream)))
+
+
+def stream_bandwidth(mode: int, dst_ptr: int | None, src_ptr: int | None, nbytes: int, blocks_per_cu: int = 8,
+                     stream=None) -> None:
+    """rsort_stream_bandwidth: one streaming launch (0 read, 1 write, 2 copy)."""
+    _check(lib().rsort_stream_bandwidth(mode, dst_ptr, src_ptr, nbytes, blocks_per_cu, _stream_ptr(stream)))
 
 
 def sort_records_raw(desc: Desc, records_ptr: int | None, n: int, record_bytes: int, key_offset: int,

@@ -118,6 +118,17 @@ rsort_status rsort_sort_profiled(const rsort_desc* desc, void* keys, void* value
                                  int* n_stages);
 
 /*
+ * Bandwidth ceiling of the device (measurement helper for benchmarks; no
+ * reference counterpart): one launch of a streaming kernel over `bytes`
+ * (a multiple of 16) in 16-byte units per lane, blocks_per_cu (1..32)
+ * 256-thread workgroups per CU.  mode 0 reads src (dst: a 4 KiB sink that
+ * is written only if the data hits an unlikely pattern), 1 writes dst,
+ * 2 copies src to dst.  Enqueued on `stream`; time it with events.
+ */
+rsort_status rsort_stream_bandwidth(uint32_t mode, void* dst, const void* src, uint64_t bytes,
+                                    uint32_t blocks_per_cu, void* stream);
+
+/*
  * Array-of-structs sort: n records of record_bytes each, sorted stably in
  * place by the key field of desc->key_bytes bytes at key_offset in every
  * record (desc->value_bytes must be 0: the rest of the record is the value).

@@ -57,6 +57,9 @@ def lib() -> ctypes.CDLL:
         L.oracle_time_stable_sort_u32.restype = ctypes.c_double
         L.oracle_time_sort_par_u32.argtypes = [vp, vp, u64, i32]
         L.oracle_time_sort_par_u32.restype = ctypes.c_double
+        L.oracle_stable_sort_par_u32.argtypes = [vp, vp, u64, i32, vp, vp]
+        L.oracle_time_stable_sort_par_u32.argtypes = [vp, vp, u64, i32, i32]
+        L.oracle_time_stable_sort_par_u32.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -222,6 +225,22 @@ def time_sort_par_u32(keys: np.ndarray, vals: np.ndarray | None, reps: int = 3) 
     k = np.ascontiguousarray(keys, dtype=np.uint32)
     v = None if vals is None else np.ascontiguousarray(vals, dtype=np.uint32)
     return float(lib().oracle_time_sort_par_u32(_p(k), _p(v), k.size, reps))
+
+
+def stable_sort_par_u32(keys: np.ndarray, vals: np.ndarray | None, threads: int):
+    """The multi-threaded stable sort of the CPU baseline (result, for its test)."""
+    k = np.ascontiguousarray(keys, dtype=np.uint32)
+    v = None if vals is None else np.ascontiguousarray(vals, dtype=np.uint32)
+    ok = np.empty_like(k)
+    ov = None if v is None else np.empty_like(v)
+    assert lib().oracle_stable_sort_par_u32(_p(k), _p(v), k.size, threads, _p(ok), _p(ov)) == 0
+    return ok, ov
+
+
+def time_stable_sort_par_u32(keys: np.ndarray, vals: np.ndarray | None, threads: int, reps: int = 3) -> float:
+    k = np.ascontiguousarray(keys, dtype=np.uint32)
+    v = None if vals is None else np.ascontiguousarray(vals, dtype=np.uint32)
+    return float(lib().oracle_time_stable_sort_par_u32(_p(k), _p(v), k.size, threads, reps))
 
 
 def pstl_parallel_backend() -> bool:

@@ -28,6 +28,7 @@
 #include <execution>
 #include <numeric>
 #include <random>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -438,6 +439,83 @@ double oracle_time_sort_par_u32(const uint32_t* keys, const uint32_t* vals, uint
             std::vector<uint32_t> k(keys, keys + n);
             const auto t0 = std::chrono::steady_clock::now();
             std::sort(std::execution::par_unseq, k.begin(), k.end());
+            t.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+        }
+    }
+    return median_of(t);
+}
+
+// Stable sort over `threads` host threads: each thread std::stable_sorts one
+// of `threads` contiguous runs, then runs are merged pairwise in rounds, the
+// merges of a round in parallel (std::merge takes from the left run on equal
+// keys, so the result is the same as one std::stable_sort).
+extern "C++" {
+template <typename R, typename Less>
+static void par_stable_sort(std::vector<R>& a, int threads, Less less) {
+    const size_t n = a.size();
+    const size_t T = threads < 1 ? 1 : (size_t)threads;
+    std::vector<size_t> b(T + 1);
+    for (size_t i = 0; i <= T; ++i) b[i] = n * i / T;
+    {
+        std::vector<std::thread> th;
+        for (size_t i = 0; i < T; ++i)
+            th.emplace_back([&, i] { std::stable_sort(a.begin() + b[i], a.begin() + b[i + 1], less); });
+        for (auto& t : th) t.join();
+    }
+    std::vector<R> tmp(n);
+    std::vector<R>* src = &a;
+    std::vector<R>* dst = &tmp;
+    for (size_t width = 1; width < T; width *= 2) {
+        std::vector<std::thread> th;
+        for (size_t i = 0; i < T; i += 2 * width) {
+            const size_t lo = b[i], mid = b[std::min(i + width, T)], hi = b[std::min(i + 2 * width, T)];
+            th.emplace_back([=] {
+                std::merge(src->begin() + lo, src->begin() + mid, src->begin() + mid, src->begin() + hi,
+                           dst->begin() + lo, less);
+            });
+        }
+        for (auto& t : th) t.join();
+        std::swap(src, dst);
+    }
+    if (src != &a) a.swap(tmp);
+}
+}  // extern "C++"
+
+// The multi-threaded stable sort of (key, value) records by key (vals NULL:
+// keys only); results to out_keys / out_vals.
+int oracle_stable_sort_par_u32(const uint32_t* keys, const uint32_t* vals, uint64_t n, int threads,
+                               uint32_t* out_keys, uint32_t* out_vals) {
+    if (vals) {
+        std::vector<std::pair<uint32_t, uint32_t>> rec(n);
+        for (uint64_t i = 0; i < n; ++i) rec[i] = {keys[i], vals[i]};
+        par_stable_sort(rec, threads, [](const auto& x, const auto& y) { return x.first < y.first; });
+        for (uint64_t i = 0; i < n; ++i) {
+            out_keys[i] = rec[i].first;
+            out_vals[i] = rec[i].second;
+        }
+    } else {
+        std::vector<uint32_t> k(keys, keys + n);
+        par_stable_sort(k, threads, [](uint32_t x, uint32_t y) { return x < y; });
+        std::copy(k.begin(), k.end(), out_keys);
+    }
+    return 0;
+}
+
+// Median seconds of the multi-threaded stable sort over reps.
+double oracle_time_stable_sort_par_u32(const uint32_t* keys, const uint32_t* vals, uint64_t n, int threads,
+                                       int reps) {
+    std::vector<double> t;
+    for (int r = 0; r < reps; ++r) {
+        if (vals) {
+            std::vector<std::pair<uint32_t, uint32_t>> rec(n);
+            for (uint64_t i = 0; i < n; ++i) rec[i] = {keys[i], vals[i]};
+            const auto t0 = std::chrono::steady_clock::now();
+            par_stable_sort(rec, threads, [](const auto& x, const auto& y) { return x.first < y.first; });
+            t.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+        } else {
+            std::vector<uint32_t> k(keys, keys + n);
+            const auto t0 = std::chrono::steady_clock::now();
+            par_stable_sort(k, threads, [](uint32_t x, uint32_t y) { return x < y; });
             t.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
         }
     }

@@ -315,3 +315,18 @@ def test_segmented_oracle_planted_mistake_is_caught(orc):
     bad, _ = orc.segmented_sort(keys, None, [0, 4, 6], 0, 4, False)
     assert good.tolist() == [3, 4, 5, 0, 1, 2]
     assert not np.array_equal(good, bad)
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8, 16])
+def test_parallel_stable_sort_baseline_is_stable(orc, threads):
+    """The CPU baseline's multi-threaded stable sort gives std::stable_sort's
+    result (equal keys keep their order across the merged runs)."""
+    rng = np.random.default_rng(threads)
+    n = 200_003
+    keys = rng.integers(0, 5000, size=n, dtype=np.uint32)  # many equal keys
+    vals = np.arange(n, dtype=np.uint32)
+    ek, ev = orc.sort(keys, vals, 0, 4, False, vb=4)
+    pk, pv = orc.stable_sort_par_u32(keys, vals, threads)
+    assert np.array_equal(pk, ek.view(np.uint32)) and np.array_equal(pv, ev.view(np.uint32))
+    kk, _ = orc.stable_sort_par_u32(keys, None, threads)
+    assert np.array_equal(kk, np.sort(keys))
