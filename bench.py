@@ -364,9 +364,13 @@ def bench_cold(rs):
             "checked": bool(ok)}
 
 
+STREAM_FORMS = {0: "strided", 4: "strided, non-temporal", 8: "chunked", 12: "chunked, non-temporal"}
+
+
 def stream_ceiling(rs, nbytes=1 << 31, reps=5):
     """Hand-written streaming kernels (rsort_stream_bandwidth): GB/s of a read,
-    a write and a copy of nbytes, best over workgroups per CU in {4, 8, 16}."""
+    a write and a copy of nbytes, best over the access forms and workgroups
+    per CU in {2, 4, 8, 16, 32}."""
     import torch
 
     dev = torch.device("cuda:0")
@@ -375,22 +379,23 @@ def stream_ceiling(rs, nbytes=1 << 31, reps=5):
     sink = torch.empty(4096, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
     out = {"bytes": nbytes}
-    for name, mode, moved in (("read", 0, nbytes), ("write", 1, nbytes), ("copy", 2, 2 * nbytes)):
-        best = (0.0, 0)
-        for bpc in (4, 8, 16):
-            dptr = sink.data_ptr() if mode == 0 else dst.data_ptr()
-            rs.stream_bandwidth(mode, dptr, src.data_ptr(), nbytes, bpc, stream)
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            for _ in range(reps):
-                rs.stream_bandwidth(mode, dptr, src.data_ptr(), nbytes, bpc, stream)
-            e1.record(stream)
-            torch.cuda.synchronize()
-            gbs = moved * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
-            if gbs > best[0]:
-                best = (gbs, bpc)
+    for name, op, moved in (("read", 0, nbytes), ("write", 1, nbytes), ("copy", 2, 2 * nbytes)):
+        best = (0.0, 0, 0)
+        for form in STREAM_FORMS:
+            for bpc in (2, 4, 8, 16, 32):
+                dptr = sink.data_ptr() if op == 0 else dst.data_ptr()
+                rs.stream_bandwidth(op | form, dptr, src.data_ptr(), nbytes, bpc, stream)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(reps):
+                    rs.stream_bandwidth(op | form, dptr, src.data_ptr(), nbytes, bpc, stream)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                gbs = moved * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+                if gbs > best[0]:
+                    best = (gbs, form, bpc)
         out[name + "_gbs"] = round(best[0], 1)
-        out[name + "_blocks_per_cu"] = best[1]
+        out[name + "_form"] = f"{STREAM_FORMS[best[1]]}, {best[2]} workgroups per CU"
     del src, dst
     torch.cuda.empty_cache()
     return out
