@@ -35,7 +35,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 # (key bytes, value bytes) tuned by default: the widths with tile candidates
-TUNED = [(4, 0), (4, 4), (8, 0), (8, 8)]
+TUNED = [(4, 0), (4, 4), (8, 0), (8, 8), (4, 8), (8, 4), (16, 0), (16, 8)]
+# flat sorts are measured at 2^log2n keys, fewer where one array would pass 2 GiB
+ARRAY_BYTES_MAX = 1 << 31
 # pass-kernel tile candidates (kTileCands in csrc/rsort_tuning.hpp)
 TILE_CANDIDATES = 4
 
@@ -168,6 +170,7 @@ def main():
     ap.add_argument("--segments", type=int, default=20000, help="segments per segmented measurement")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default=os.path.join(HERE, "tuning_gfx950.txt"))
+    ap.add_argument("--widths", default=None, help="key:value byte pairs to tune, e.g. 4:4,16:8 (default: all)")
     a = ap.parse_args()
     import torch
 
@@ -175,10 +178,14 @@ def main():
 
     assert torch.cuda.is_available(), "the autotuner measures on a GPU"
     rs.tuning_load("")  # start from the built-in defaults
-    n = 1 << a.log2n
     sizes = lognormal_sizes(a.segments)
     lines, report = [], []
-    for kb, vb in TUNED:
+    widths = TUNED if not a.widths else [tuple(int(x) for x in w.split(":")) for w in a.widths.split(",")]
+    for kb, vb in widths:
+        log2n = a.log2n
+        while (kb + vb) << log2n > ARRAY_BYTES_MAX:
+            log2n -= 1
+        n = 1 << log2n
         base = rs.tuning_get(kb, vb)
         caps = list(rs.segmented_plan(rs.make_desc(0, kb, vb), 1 << 20, 8, 1).bin_capacity)
         # tile candidates (histogram setting fixed at its default)
@@ -201,7 +208,7 @@ def main():
         best_cap = min(seg_ms, key=seg_ms.get)
         # tile candidate of small sorts
         small_ms = {}
-        for l2 in [s for s in (20, 21, 22, 23, 24, 26) if s < a.log2n]:
+        for l2 in [s for s in (20, 21, 22, 23, 24, 26) if s < log2n]:
             row = {}
             for c in range(TILE_CANDIDATES):
                 rs.tuning_set(kb, vb, c, best_bpc, best_cap)
@@ -210,7 +217,7 @@ def main():
         tiny_cand, tiny_log2n, small_cand, small_log2n = small_choice(small_ms, best_tile)
         rs.tuning_set(kb, vb, best_tile, best_bpc, best_cap, small_cand, small_log2n, tiny_cand, tiny_log2n)
         lines.append(f"{kb} {vb} {best_tile} {best_bpc} {best_cap} {small_cand} {small_log2n} {tiny_cand} {tiny_log2n}")
-        report.append({"key_bytes": kb, "value_bytes": vb, "pass_ms_by_tile_candidate": tile_ms,
+        report.append({"key_bytes": kb, "value_bytes": vb, "log2n": log2n, "pass_ms_by_tile_candidate": tile_ms,
                        "histogram_ms_by_blocks_per_cu": hist_ms, "segmented_ms_by_lds_threshold": seg_ms,
                        "sort_ms_by_log2n_and_candidate": small_ms,
                        "chosen": {"tile_candidate": best_tile, "hist_blocks_per_cu": best_bpc,
@@ -220,7 +227,8 @@ def main():
         print(json.dumps(report[-1]), flush=True)
     props = torch.cuda.get_device_properties(0)
     header = [f"# rsort tuning table for {getattr(props, 'gcnArchName', 'gfx950')} ({props.name}), written by autotune.py",
-              f"# {time.strftime('%Y-%m-%d')}: flat sorts of 2^{a.log2n} uniform keys, segmented sorts of "
+              f"# {time.strftime('%Y-%m-%d')}: flat sorts of 2^{a.log2n} uniform keys (fewer where an array "
+              f"would pass {ARRAY_BYTES_MAX >> 30} GiB), segmented sorts of "
               f"{a.segments} lognormal segments, mean of {a.reps} runs each",
               "# key_bytes value_bytes tile_candidate hist_blocks_per_cu lds_threshold small_tile_candidate small_log2n "
               "tiny_tile_candidate tiny_log2n"]

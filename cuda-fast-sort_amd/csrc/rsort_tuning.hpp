@@ -154,20 +154,22 @@ RSA_HD constexpr int pass_min_waves(int key_bytes, int val_bytes, int items) {
 }
 
 // Tile candidates of the pass kernel the autotuner chooses from.  Candidate
-// 0 is tile_cfg; 1, 2 and 3 (four items fewer per thread; half the threads;
-// half the items) are built for the widths sorted most (4- and 8-byte keys,
-// keys only or with values of the key's width) and fall back to candidate 0
-// elsewhere.
+// 0 is tile_cfg; 1 has fewer items per thread (four fewer from 12 items up,
+// else two fewer), 2 half the threads, 3 half the items.  They are built for
+// the widths sorted most -- 4- and 8-byte keys alone or with 4- or 8-byte
+// values, 16-byte keys alone or with 8-byte values -- and fall back to
+// candidate 0 elsewhere.
 constexpr int kTileCands = 4;
 
 RSA_HD constexpr bool tile_cands_built(int key_bytes, int val_bytes) {
-    return (key_bytes == 4 || key_bytes == 8) && (val_bytes == 0 || val_bytes == key_bytes);
+    return ((key_bytes == 4 || key_bytes == 8) && (val_bytes == 0 || val_bytes == 4 || val_bytes == 8)) ||
+           (key_bytes == 16 && (val_bytes == 0 || val_bytes == 8));
 }
 
 RSA_HD constexpr TileCfg tile_cand(int key_bytes, int val_bytes, int cand) {
     const TileCfg d = tile_cfg(key_bytes, val_bytes);
     if (!tile_cands_built(key_bytes, val_bytes) || cand <= 0 || cand >= kTileCands) return d;
-    if (cand == 1) return TileCfg{d.threads, d.items - 4u};
+    if (cand == 1) return TileCfg{d.threads, d.items >= 12u ? d.items - 4u : d.items - 2u};
     if (cand == 2) return TileCfg{d.threads / 2u, d.items};
     return TileCfg{d.threads, d.items / 2u};
 }

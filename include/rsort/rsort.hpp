@@ -32,9 +32,21 @@ namespace detail {
 // Placeholder value type of key-only sorts (the reference's no_value_t).
 struct no_value_t {};
 
+// IEEE half keys (_Float16), where the compiler has the type in C++ (clang,
+// hipcc's host compiler; gcc 12 and later on x86-64).  The reference takes
+// arithmetic types only (tuning.cuh:301-304); half keys are an addition, also
+// reachable through the C ABI (RSORT_KEY_FLOAT, 2 bytes) with any compiler.
+template <typename T>
+struct is_float16 : std::false_type {};
+#if defined(__FLT16_MAX__)
+template <>
+struct is_float16<_Float16> : std::true_type {};
+#endif
+
 // Key kind / width of Key_T (radix_traits, include/rsort/detail/tuning.cuh:297-397).
 template <typename T>
 struct key_desc {
+    static constexpr bool is_half = is_float16<T>::value;
     static constexpr bool is_ld = std::is_same_v<T, long double> && sizeof(long double) == 16;
 #if defined(__SIZEOF_INT128__)
     static constexpr bool is_u128 = std::is_same_v<T, unsigned __int128>;
@@ -43,11 +55,11 @@ struct key_desc {
     static constexpr bool is_u128 = false;
     static constexpr bool is_i128 = false;
 #endif
-    static_assert(std::is_arithmetic_v<T> || is_u128 || is_i128, "Radix type must be a numeric.");
+    static_assert(std::is_arithmetic_v<T> || is_u128 || is_i128 || is_half, "Radix type must be a numeric.");
     static_assert(sizeof(T) <= 16, "Radix type too large.");
     static_assert(!std::is_floating_point_v<T> || sizeof(T) != 1, "no 1-byte floating keys");
     static constexpr uint32_t bytes = (uint32_t)sizeof(T);
-    static constexpr uint32_t kind = (std::is_floating_point_v<T> || is_ld) ? RSORT_KEY_FLOAT
+    static constexpr uint32_t kind = (std::is_floating_point_v<T> || is_ld || is_half) ? RSORT_KEY_FLOAT
                                      : ((std::is_signed_v<T> || is_i128) ? RSORT_KEY_SIGNED : RSORT_KEY_UNSIGNED);
 };
 

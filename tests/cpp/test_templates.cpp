@@ -44,7 +44,10 @@ static bool less_key(const K& a, const K& b) {
 
 template <typename K>
 static K make_key(std::mt19937_64& g) {
-    if constexpr (std::is_floating_point_v<K>) {
+    if constexpr (rsort::detail::is_float16<K>::value) {
+        // multiples of 1/8 in [-125, 125]: exact in half precision, many duplicates
+        return (K)(((double)(int64_t)(g() % 2001) - 1000.0) / 8.0);
+    } else if constexpr (std::is_floating_point_v<K>) {
         K v;
         memset(&v, 0, sizeof(K));
         v = (K)((double)(int64_t)(g() % 2001) - 1000.0) / (K)8;  // duplicates and +-values
@@ -165,6 +168,11 @@ static void run_records(const char* name, size_t n) {
 }
 
 int main() {
+#if defined(__FLT16_MAX__)
+    printf("compiler has _Float16: half keys tested\n");
+#else
+    printf("compiler without _Float16 in C++: half keys through the C ABI only\n");
+#endif
     for (size_t n : {(size_t)1000, (size_t)123457}) {
         run_case<uint32_t, uint32_t, false, false>("sort<uint32_t>", n);
         run_case<int32_t, uint32_t, true, false>("sort_descending<int32_t>", n);
@@ -177,6 +185,11 @@ int main() {
         run_case<unsigned __int128, uint64_t, false, true>("sort_pairs<unsigned __int128,uint64_t>", n);
         run_case<__int128, uint32_t, true, true>("sort_pairs_descending<__int128,uint32_t>", n);
         run_case<long double, uint32_t, false, true>("sort_pairs<long double,uint32_t>", n);
+#if defined(__FLT16_MAX__)
+        // half keys through the templates (compilers with _Float16 in C++)
+        run_case<_Float16, uint32_t, false, true>("sort_pairs<_Float16,uint32_t>", n);
+        run_case<_Float16, uint32_t, true, false>("sort_descending<_Float16>", n);
+#endif
     }
     for (size_t n : {(size_t)1000, (size_t)200003}) {
         run_records<false>("sort_records<Particle, &Particle::y>", n);

@@ -9,7 +9,7 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-@pytest.mark.parametrize("name", ["test_templates"])
+@pytest.mark.parametrize("name", ["test_templates", "test_templates_clang"])
 def test_cpp_program(name):
     exe = os.path.join(ROOT, "tests", "cpp", "_build", name)
     assert os.path.exists(exe), f"{exe} not built (run __graft_entry__.build())"

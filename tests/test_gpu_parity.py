@@ -230,7 +230,7 @@ def test_python_api_mirror(rs, orc):
     np.testing.assert_array_equal(y.cpu().numpy(), ref)
 
 
-@pytest.mark.parametrize("kb,vb", [(4, 0), (4, 4), (8, 0), (8, 8)])
+@pytest.mark.parametrize("kb,vb", [(4, 0), (4, 4), (8, 0), (8, 8), (4, 8), (8, 4), (16, 0), (16, 8)])
 @pytest.mark.parametrize("cand", [0, 1, 2, 3])
 def test_tile_candidates(rs, orc, kb, vb, cand):
     """Every pass-kernel tile candidate the autotuner may choose, at every
