@@ -9,6 +9,7 @@ run also measures, each in an object of the line:
   hbm_ceiling    hand-written read / write / copy kernels (the practical ceiling)
   configs        plain 2^20 u32 (mt19937 seed 0) latency, 2^27 u64 low-16 and
                  Zipf, 2^27 f32 with +-0/+-inf, 2^26 u128/u64 pairs
+  size_sweep     u32 keys and u32/u32 pairs at n = 2^20, 2^22, ..., 2^30
   byte_skip      2^27 u32 from the reference generator, every other byte constant
   cold_vs_steady 2^27 u32: the process's first sort and cache-flushed sorts
   segmented      100k lognormal segments (~2^28 u32/u32 keys) in one graph
@@ -55,11 +56,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["kv", "keys", "copy", "segments", "serve", *CONFIG_WORKLOADS], default="kv",
+    ap.add_argument("--workload", choices=["kv", "keys", "copy", "segments", "serve", "sweep", *CONFIG_WORKLOADS],
+                    default="kv",
                     help="kv: sort_pairs u32/u32 (the full line); keys: sort u32; copy: the streaming kernels "
                          "(bandwidth ceiling, and the known-size kernel that calibrates HBM counters); "
                          "segments: 100k lognormal-sized u32/u32 segments, graph replay vs one call per segment; "
-                         "serve: Poisson request stream through the serving loop")
+                         "serve: Poisson request stream through the serving loop; "
+                         "sweep: u32 keys and pairs at n = 2^20..2^30")
     ap.add_argument("--segments", type=int, default=100_000)
     ap.add_argument("--per-call-sample", type=int, default=20_000,
                     help="segments sorted one rsort_sort call each for the per-call comparison")
@@ -289,6 +292,60 @@ def bench_plain20(rs, steps, warmup):
         out[label] = {"us_per_sort_alone_median": round(alone[len(alone) // 2], 2),
                       "us_per_sort_back_to_back": round(b2b, 2), "checked": bool(ok)}
     out["us_per_sort"] = out["direct"]["us_per_sort_alone_median"]
+    return out
+
+
+SWEEP_LOG2N = (20, 22, 24, 26, 28, 30)
+
+
+def bench_size_sweep(rs, sizes=SWEEP_LOG2N):
+    """Throughput over n = 2^20..2^30, u32 keys and u32/u32 pairs (index
+    values), uniform keys: each call timed alone with events around it (the
+    input restored from a copy outside the events), median over the calls;
+    GKeys/s and the formula roofline (1 + passes) * 2 * n * (key + value
+    bytes) over 8 TB/s.  Every size's last result is checked on the device
+    (order, digit histograms, value pairing)."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    out = {"description": "u32 keys uniform over [0, 2^32), values = index; whole call, device time, median of "
+                          "calls timed alone", "keys": {}, "pairs": {}}
+    for label, vb in (("keys", 0), ("pairs", 4)):
+        for log2n in sizes:
+            n = 1 << log2n
+            keys0 = gen_u32(n, dev, log2n)
+            vals0 = torch.arange(n, dtype=torch.int32, device=dev) if vb else None
+            d = rs.make_desc(0, 4, vb)
+            tb = rs.sort_raw(d, None, None, None, 0, n)
+            ws = torch.empty(tb, dtype=torch.uint8, device=dev)
+            k = keys0.clone()
+            v = vals0.clone() if vb else None
+            calls = max(3, min(20, (1 << 26) // n))
+            times = []
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            for c in range(calls + 2):  # two warm-up calls
+                k.copy_(keys0)
+                if vb:
+                    v.copy_(vals0)
+                torch.cuda.synchronize()
+                e0.record(stream)
+                rs.sort_raw(d, k.data_ptr(), v.data_ptr() if vb else None, ws.data_ptr(), tb, n, stream)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                if c >= 2:
+                    times.append(e0.elapsed_time(e1))
+            ok = rs.check_sorted(k, d, n) == 0 and torch.equal(rs.digit_counts(k, d, n), rs.digit_counts(keys0, d, n))
+            if vb:
+                ok = ok and rs.check_pairings(keys0, k, v, 4, 4, n) == 0
+            times.sort()
+            ms = times[len(times) // 2]
+            formula = 5 * 2.0 * n * (4 + vb)
+            out[label][f"2^{log2n}"] = {"gkeys_per_s": round(n / (ms * 1e-3) / 1e9, 3), "us_per_sort": round(ms * 1e3, 2),
+                                        "formula_frac": round(formula / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                                        "calls": calls, "checked": bool(ok)}
+            del keys0, vals0, k, v, ws
+            torch.cuda.empty_cache()
     return out
 
 
@@ -700,6 +757,9 @@ def main():
     if a.workload == "serve":
         print(json.dumps(bench_serve(rs, a)), flush=True)
         return
+    if a.workload == "sweep":
+        print(json.dumps(bench_size_sweep(rs)), flush=True)
+        return
     if a.workload in CONFIG_WORKLOADS:
         r = bench_plain20(rs, a.steps, a.warmup) if a.workload == "plain20" else \
             bench_config(rs, a.workload, a.steps, a.warmup, a.descending, rs.FLAG_GRAPH if a.graph else 0)
@@ -738,7 +798,9 @@ def main():
             out["configs"][name] = bench_config(rs, name, max(3, a.steps // 2), 2)
             print(f"[bench] {name} done", file=sys.stderr, flush=True)
         out["plain20_us"] = out["configs"]["plain20"]["us_per_sort"]
-        rnd = bench_init_keys_mode(rs, 1, 1 << 27, max(3, a.steps // 2), 2)
+        out["size_sweep"] = bench_size_sweep(rs)
+        print("[bench] size sweep done", file=sys.stderr, flush=True)
+        rnd =bench_init_keys_mode(rs, 1, 1 << 27, max(3, a.steps // 2), 2)
         bsk = bench_init_keys_mode(rs, 2, 1 << 27, max(3, a.steps // 2), 2)
         out["byte_skip"] = {"description": "u32 n=2^27 from the reference generator: random vs byte_skip (every "
                                            "other byte constant), steady state", "random": rnd, "byte_skip": bsk}
