@@ -1,8 +1,8 @@
 """The benchmark / validation driver cuda-fast-sort_amd/cli/radix_gpu, with the
 reference driver's options (benchmark/bench_parser.cpp:52-106): argument
-handling on the CPU; on the GPU a reduced validation matrix (every key type x
-value type x mode x order, sizes 10, 100, 1000 and 2^10 .. 2^20) and a short
-benchmark run, both checked on the device."""
+handling on the CPU; on the GPU the validation matrix (every key type x value
+type x mode x order, sizes 10, 100, 1000 and 2^10 .. 2^20, then 2^17 .. 2^27)
+and a short benchmark run, all checked on the device."""
 from __future__ import annotations
 
 import os
@@ -37,14 +37,19 @@ def test_bad_arguments(args):
 
 
 @pytest.mark.gpu
-def test_validation_matrix():
-    r = run("--validation", "--n", str(1 << 20), timeout=600)
+@pytest.mark.parametrize("log2n", [20, 27])
+def test_validation_matrix(log2n):
+    """The reference's matrix up to 2^20, and up to its default size 2^27
+    (sizes 10, 100, 1000, 2^17 .. 2^27), the size of the reference's own
+    validation log (data/logs/validation_log.txt:6557)."""
+    r = run("--validation", "--n", str(1 << log2n), timeout=600)
     print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
     m = re.search(r"Done\.\.\. passed (\d+)/(\d+) tests\.", r.stdout)
     assert m and m.group(1) == m.group(2)
     assert int(m.group(2)) == 13 * 6 * 3 * 2 * 14
     assert "FAILED" not in r.stdout
+    assert f"(10 .. 2^{log2n})" in r.stdout
 
 
 @pytest.mark.gpu
