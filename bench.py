@@ -663,6 +663,18 @@ def bench_serve(rs, a):
     }
 
 
+def host_cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(keys0, with_values, log2n):
     """Stable sorts of the same (key, index) records on the host cores:
     std::stable_sort on one core, and the oracle's multi-threaded stable sort
@@ -685,6 +697,8 @@ def cpu_baseline(keys0, with_values, log2n):
         "single_core": {"value": round(m / t_one / 1e9, 5), "unit": "GKeys/s", "cores": 1,
                         "sample": f"std::stable_sort of the same 2^{log2n} {what}, one run"},
         "host_cores_available": cores,
+        "host_cpu": host_cpu_model(),
+        "pstl_backend": "parallel (TBB)" if oracle.pstl_parallel_backend() else "serial (libstdc++ without TBB)",
     }
     if oracle.pstl_parallel_backend():
         t_pu = oracle.time_sort_par_u32(sample_k, sample_v, 1)
