@@ -6,7 +6,8 @@ step = one complete sort of a fresh copy of that input (inputs are resident in
 HBM before the timed region; each step sorts its own copy in place).  The same
 run also measures, each in an object of the line:
   key_only       the north-star sort: 2^28 u32 keys, with its pass roofline
-  hbm_ceiling    hand-written read / write / copy kernels (the practical ceiling)
+  hbm_ceiling    hand-written read / write / copy kernels, and a scatter copy with
+                 one radix pass's write pattern (the practical ceilings)
   configs        plain 2^20 u32 (mt19937 seed 0) latency, 2^27 u64 low-16 and
                  Zipf, 2^27 f32 with +-0/+-inf, 2^26 u128/u64 pairs
   size_sweep     u32 keys and u32/u32 pairs at n = 2^20, 2^22, ..., 2^30
@@ -426,8 +427,10 @@ STREAM_FORMS = {0: "strided", 4: "strided, non-temporal", 8: "chunked", 12: "chu
 
 def stream_ceiling(rs, nbytes=1 << 31, reps=5):
     """Hand-written streaming kernels (rsort_stream_bandwidth): GB/s of a read,
-    a write and a copy of nbytes, best over the access forms and workgroups
-    per CU in {2, 4, 8, 16, 32}."""
+    a write, a copy and a scatter copy (the write pattern of one radix pass:
+    64 KiB tiles of 4-byte keys written as 256 runs of 64 keys to 256
+    regions) of nbytes, best over the access forms and workgroups per CU in
+    {2, 4, 8, 16, 32}."""
     import torch
 
     dev = torch.device("cuda:0")
@@ -436,9 +439,10 @@ def stream_ceiling(rs, nbytes=1 << 31, reps=5):
     sink = torch.empty(4096, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
     out = {"bytes": nbytes}
-    for name, op, moved in (("read", 0, nbytes), ("write", 1, nbytes), ("copy", 2, 2 * nbytes)):
+    for name, op, moved in (("read", 0, nbytes), ("write", 1, nbytes), ("copy", 2, 2 * nbytes),
+                            ("scatter", 3, 2 * nbytes)):
         best = (0.0, 0, 0)
-        for form in STREAM_FORMS:
+        for form in (STREAM_FORMS if op != 3 else (0, 4)):
             for bpc in (2, 4, 8, 16, 32):
                 dptr = sink.data_ptr() if op == 0 else dst.data_ptr()
                 rs.stream_bandwidth(op | form, dptr, src.data_ptr(), nbytes, bpc, stream)
@@ -451,21 +455,33 @@ def stream_ceiling(rs, nbytes=1 << 31, reps=5):
                 gbs = moved * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
                 if gbs > best[0]:
                     best = (gbs, form, bpc)
+                if op == 3 and form == 0:  # plain stores, as the sort's scatter uses
+                    out["scatter_plain_gbs"] = round(max(out.get("scatter_plain_gbs", 0.0), gbs), 1)
         out[name + "_gbs"] = round(best[0], 1)
-        out[name + "_form"] = f"{STREAM_FORMS[best[1]]}, {best[2]} workgroups per CU"
+        out[name + "_form"] = (f"{STREAM_FORMS[best[1]]}, {best[2]} workgroups per CU" if op != 3 else
+                               f"{'non-temporal' if best[1] else 'plain'}, {best[2]} workgroups per CU")
     del src, dst
     torch.cuda.empty_cache()
     return out
 
 
+# The sources a plain sort is built from: its kernels, their launch and the
+# host dispatch (not the segmented sort, the server, the device checkers or
+# the streaming kernels).
+SORT_SOURCES = ("rsort_host.hip", "rsort_kernels.hpp", "rsort_launch.hpp", "rsort_launch_impl.hpp",
+                "rsort_onesweep_tile.inc", "rsort_protocol.hpp", "rsort_tuning.hpp")
+
+
 def kernel_sources_sha256():
-    """Hash of the library sources (every file under csrc/): a traffic figure
-    measured on other sources is not reported."""
+    """Hash of the sources a plain sort is built from (SORT_SOURCES and the
+    sort_kb*.hip instantiation units under csrc/): a traffic figure measured
+    on other sources is not reported."""
     h = hashlib.sha256()
     csrc = os.path.join(PKG, "csrc")
     for f in sorted(os.listdir(csrc)):
-        with open(os.path.join(csrc, f), "rb") as fh:
-            h.update(f.encode() + b"\0" + fh.read())
+        if f in SORT_SOURCES or (f.startswith("sort_kb") and f.endswith(".hip")):
+            with open(os.path.join(csrc, f), "rb") as fh:
+                h.update(f.encode() + b"\0" + fh.read())
     return h.hexdigest()
 
 
@@ -804,8 +820,14 @@ def main():
         print("[bench] key-only done", file=sys.stderr, flush=True)
         out["hbm_ceiling"] = stream_ceiling(rs)
         cp = out["hbm_ceiling"]["copy_gbs"]
+        sc = out["hbm_ceiling"]["scatter_gbs"]
         out["roofline"]["frac_of_copy_ceiling"] = round(out["roofline"]["achieved"] / cp, 4)
         out["key_only"]["roofline"]["frac_of_copy_ceiling"] = round(kroof["achieved"] / cp, 4)
+        scp = out["hbm_ceiling"]["scatter_plain_gbs"]
+        out["roofline"]["frac_of_scatter_ceiling"] = round(out["roofline"]["achieved"] / sc, 4)
+        out["key_only"]["roofline"]["frac_of_scatter_ceiling"] = round(kroof["achieved"] / sc, 4)
+        out["roofline"]["frac_of_plain_scatter_ceiling"] = round(out["roofline"]["achieved"] / scp, 4)
+        out["key_only"]["roofline"]["frac_of_plain_scatter_ceiling"] = round(kroof["achieved"] / scp, 4)
         print("[bench] ceiling done", file=sys.stderr, flush=True)
         out["configs"] = {"plain20": bench_plain20(rs, a.steps, a.warmup)}
         for name in ("u64low16", "zipf", "f32", "u128kv"):

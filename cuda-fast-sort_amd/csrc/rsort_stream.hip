@@ -12,6 +12,13 @@
 // Measured on MI355X (copy of 2 GiB): chunked with non-temporal accesses
 // 5.4-5.5 TB/s, chunked plain 5.2-5.3, strided 4.2-4.7 (and a one-unit
 // strided copy up to 5.5 at 4 workgroups per CU).
+//
+// A third form, the scatter copy, has the write pattern of one LSD pass: the
+// buffer is tiles of 16384 4-byte keys (64 KiB), read in order, and tile t is
+// written as 256 runs of 64 keys, run d at position t * 64 of bucket d's
+// region (256 regions of tiles * 64 keys, one after another).  It is the pass
+// of a sort whose 256 digits are equally frequent in every tile, an exact
+// permutation with no ranking: the ceiling of a pass's own access pattern.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -97,6 +104,27 @@ __global__ __launch_bounds__(256) void k_stream(uint4* __restrict__ dst, const u
     }
 }
 
+// Scatter copy: one 256-thread workgroup per tile (grid-stride over tiles),
+// 16 units of 16 bytes per lane, all loads in flight before the stores.
+// Sixteen consecutive lanes write one run (256 contiguous bytes).
+template <bool NT>
+__global__ __launch_bounds__(256) void k_scatter(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                                 uint64_t tiles) {
+    constexpr int U = 16;
+    const uint64_t region = tiles * 16u;  // 16-byte units per bucket region
+    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint4* s = src + t * 4096u;
+        uint4 q[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) q[k] = ld16<NT>(s + k * 256 + threadIdx.x);
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const uint32_t u = (uint32_t)k * 256u + threadIdx.x;
+            st16<NT>(dst + (uint64_t)(u >> 4) * region + t * 16u + (u & 15u), q[k]);
+        }
+    }
+}
+
 template <int OP>
 void launch_op(uint32_t form, dim3 grid, hipStream_t s, uint4* dst, const uint4* src, uint64_t units) {
     switch (form) {
@@ -113,9 +141,10 @@ void launch_op(uint32_t form, dim3 grid, hipStream_t s, uint4* dst, const uint4*
 extern "C" rsort_status rsort_stream_bandwidth(uint32_t mode, void* dst, const void* src, uint64_t bytes,
                                                uint32_t blocks_per_cu, void* stream) {
     const uint32_t op = mode & 3u, form = (mode >> 2) & 3u;
-    if (op > 2 || (mode & ~15u) || (bytes & 15u) || (((uintptr_t)dst | (uintptr_t)src) & 15u) ||
-        blocks_per_cu < 1 || blocks_per_cu > 32)
+    if ((mode & ~15u) || (bytes & 15u) || (((uintptr_t)dst | (uintptr_t)src) & 15u) || blocks_per_cu < 1 ||
+        blocks_per_cu > 32)
         return RSORT_ERROR_INVALID_ARGUMENT;
+    if (op == 3 && ((bytes & 65535u) || (form & 2u) || dst == src)) return RSORT_ERROR_INVALID_ARGUMENT;
     if (!dst || (op != 1 && !src)) return RSORT_ERROR_INVALID_ARGUMENT;
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -126,6 +155,10 @@ extern "C" rsort_status rsort_stream_bandwidth(uint32_t mode, void* dst, const v
     hipStream_t s = (hipStream_t)stream;
     if (op == 0) rsa::launch_op<0>(form, grid, s, (uint4*)dst, (const uint4*)src, units);
     else if (op == 1) rsa::launch_op<1>(form, grid, s, (uint4*)dst, (const uint4*)src, units);
-    else rsa::launch_op<2>(form, grid, s, (uint4*)dst, (const uint4*)src, units);
+    else if (op == 2) rsa::launch_op<2>(form, grid, s, (uint4*)dst, (const uint4*)src, units);
+    else if (form & 1u)
+        hipLaunchKernelGGL((rsa::k_scatter<true>), grid, dim3(256), 0, s, (uint4*)dst, (const uint4*)src, bytes >> 16);
+    else
+        hipLaunchKernelGGL((rsa::k_scatter<false>), grid, dim3(256), 0, s, (uint4*)dst, (const uint4*)src, bytes >> 16);
     return hipGetLastError() == hipSuccess ? RSORT_SUCCESS : RSORT_ERROR_DEVICE;
 }

@@ -245,7 +245,7 @@ def sort_noalloc_raw(desc: Desc, keys_ptr: int | None, vals_ptr: int | None, key
 
 def stream_bandwidth(mode: int, dst_ptr: int | None, src_ptr: int | None, nbytes: int, blocks_per_cu: int = 8,
                      stream=None) -> None:
-    """rsort_stream_bandwidth: one streaming launch (0 read, 1 write, 2 copy)."""
+    """rsort_stream_bandwidth: one streaming launch (0 read, 1 write, 2 copy, 3 scatter copy)."""
     _check(lib().rsort_stream_bandwidth(mode, dst_ptr, src_ptr, nbytes, blocks_per_cu, _stream_ptr(stream)))
 
 

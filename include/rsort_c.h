@@ -123,9 +123,13 @@ rsort_status rsort_sort_profiled(const rsort_desc* desc, void* keys, void* value
  * (a multiple of 16) in 16-byte units per lane, blocks_per_cu (1..32)
  * 256-thread workgroups per CU.  mode bits 0-1: 0 reads src (dst: a 4 KiB
  * sink that is written only if the data hits an unlikely pattern), 1 writes
- * dst, 2 copies src to dst; bit 2: non-temporal loads and stores; bit 3:
- * each workgroup takes contiguous 32 KiB chunks (else every lane strides by
- * the grid's width).  Enqueued on `stream`; time it with events.
+ * dst, 2 copies src to dst, 3 scatter-copies src to a separate dst (bytes a
+ * multiple of 64 KiB: each 64 KiB tile of 4-byte keys is written as 256 runs
+ * of 64 keys, run d at position tile * 64 of the d-th of 256 equal regions,
+ * the write pattern of one radix pass); bit 2: non-temporal loads and
+ * stores; bit 3 (modes 0-2): each workgroup takes contiguous 32 KiB chunks
+ * (else every lane strides by the grid's width).  Enqueued on `stream`; time
+ * it with events.
  */
 rsort_status rsort_stream_bandwidth(uint32_t mode, void* dst, const void* src, uint64_t bytes,
                                     uint32_t blocks_per_cu, void* stream);

@@ -398,3 +398,19 @@ def test_noalloc_rejects_before_device_work(rs):
     # trivial sizes are no-ops that touch nothing
     for (kb, vb), n in ok_sizes:
         rs.sort_noalloc_raw(rs.make_desc(0, kb, vb), fake, fake, fake * 2, fake * 2, 1)
+
+
+@pytest.mark.parametrize("mode,nbytes,bpc,same", [
+    (3, 65536 + 16, 4, False),   # scatter: not whole 64 KiB tiles
+    (3 | 8, 65536, 4, False),    # scatter has no chunked form
+    (3, 65536, 4, True),         # scatter in place
+    (16, 4096, 4, False),        # unknown mode bit
+    (2, 4096 + 8, 4, False),     # not whole 16-byte units
+    (2, 4096, 0, False),         # blocks per CU out of range
+    (2, 4096, 33, False),
+])
+def test_stream_bandwidth_rejects_bad_arguments(rs, mode, nbytes, bpc, same):
+    """The streaming kernels check their arguments before any device call."""
+    with pytest.raises(rs.RsortError) as e:
+        rs.stream_bandwidth(mode, 0x100000, 0x100000 if same else 0x200000, nbytes, bpc)
+    assert e.value.status == 1
