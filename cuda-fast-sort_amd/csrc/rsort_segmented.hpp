@@ -225,7 +225,15 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
     static_assert(THREADS % 64 == 0 && THREADS <= 1024, "workgroup of whole waves");
 
     __shared__ __attribute__((aligned(16))) uint8_t s_big[BIG];
-    __shared__ __attribute__((aligned(16))) uint32_t s_cnt[WAVES * 256];
+    // Per-wave digit counters, then the waves' digit starts: both below TILE,
+    // so 16 bits hold them for bins under 64K keys (half the LDS bytes of the
+    // rank's counter reads and writes and of the exchange's offset reads;
+    // 100k-segment batch +1.4 % on MI355X)
+#ifndef RSA_SEG_CNT16
+#define RSA_SEG_CNT16 1
+#endif
+    using SCnt = std::conditional_t<(RSA_SEG_CNT16 && TILE < 65536u), uint16_t, uint32_t>;
+    __shared__ __attribute__((aligned(16))) SCnt s_cnt[WAVES * 256];
     __shared__ uint32_t s_wsum[4];
     __shared__ uint32_t s_item[2];  // by segment parity: this segment's ticket, the next one's
     __shared__ uint32_t s_uniform[2];  // by pass parity: reset two passes after its last read
@@ -299,7 +307,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
                 uint64_t* M = (uint64_t*)s_big + (size_t)w * 256;
                 for (uint32_t k = lane; k < 256u; k += 64u) {
                     M[k] = 0ull;
-                    s_cnt[w * 256 + k] = 0u;
+                    s_cnt[w * 256 + k] = (SCnt)0u;
                 }
             }
             RSA_COMPILER_FENCE();  // own-wave tables: the LDS keeps a wave's order
@@ -339,7 +347,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
                 for (int r = 0; r < R; ++r) {
                     const uint32_t d = tid * R + r;
 #pragma unroll
-                    for (int ww = 0; ww < WAVES; ++ww) s_cnt[ww * 256 + d] = base + col[r][ww];
+                    for (int ww = 0; ww < WAVES; ++ww) s_cnt[ww * 256 + d] = (SCnt)(base + col[r][ww]);
                     base += tot[r];
                 }
             }
