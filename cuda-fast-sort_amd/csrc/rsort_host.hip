@@ -714,6 +714,49 @@ rsort_status run_sort_noalloc(const rsort_desc* d, void* keys, void* values, voi
     return enqueue_sort(p, keys, values, b, n, stream, nullptr, nullptr, 0, nullptr);
 }
 
+// Side streams of the segmented sort, per device: one set for direct calls
+// (the fork and join are enqueued under a per-device lock, so that calls from
+// several host threads cannot interleave the events' records and waits) and
+// one for graph captures (used only under the graph cache's lock).  Created
+// once, non-blocking, at the default priority; kept until the process ends.
+// RSORT_AMD_SEG_STREAMS=0 keeps every launch on the caller's stream.
+struct SideSet {
+    hipStream_t s[3];
+    hipEvent_t fork, join[3];
+    bool ok;
+};
+
+bool seg_streams_allowed() {
+    static const bool v = [] {
+        const char* e = getenv("RSORT_AMD_SEG_STREAMS");
+        return !(e && strcmp(e, "0") == 0);
+    }();
+    return v;
+}
+
+SideSet* side_set(int dev, bool capture) {
+    static std::mutex mu;
+    static SideSet sets[64][2];
+    static bool done[64][2] = {{false}};
+    if (dev < 0 || dev >= 64 || !seg_streams_allowed()) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    SideSet& ss = sets[dev][capture ? 1 : 0];
+    if (!done[dev][capture ? 1 : 0]) {
+        done[dev][capture ? 1 : 0] = true;
+        hipError_t e = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming);
+        for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipStreamCreateWithFlags(&ss.s[i], hipStreamNonBlocking);
+        for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&ss.join[i], hipEventDisableTiming);
+        ss.ok = e == hipSuccess;
+        (void)hipGetLastError();
+    }
+    return ss.ok ? &ss : nullptr;
+}
+
+std::mutex& side_lock(int dev) {
+    static std::mutex mu[64];
+    return mu[(dev >= 0 && dev < 64) ? dev : 0];
+}
+
 hipError_t dispatch_seg(const SegJob& j, hipStream_t s) {
     switch (j.kb) {
         case 1: return launch_seg_kb1(j, s);
@@ -791,7 +834,23 @@ rsort_status run_segmented(const rsort_desc* d, void* keys, void* values, uint64
             e2 = launch_iota((uint32_t*)j.vals, n, cus, s);
             if (e2 != hipSuccess) return hip_fail(e2, "index fill");
         }
-        e2 = dispatch_seg(j, s);
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        (void)hipStreamIsCapturing(s, &cap);
+        SideSet* ss = events ? nullptr : side_set(dev, cap != hipStreamCaptureStatusNone);
+        SegJob js = j;
+        if (ss) {
+            for (int i = 0; i < 3; ++i) {
+                js.side[i] = ss->s[i];
+                js.join[i] = ss->join[i];
+            }
+            js.fork = ss->fork;
+            // direct calls share the set's events across host threads
+            std::unique_lock<std::mutex> lk(side_lock(dev), std::defer_lock);
+            if (cap == hipStreamCaptureStatusNone) lk.lock();
+            e2 = dispatch_seg(js, s);
+        } else {
+            e2 = dispatch_seg(js, s);
+        }
         if (e2 != hipSuccess) return hip_fail(e2, "segmented sort launch");
         if (p.index_mode) {
             void* copy = base + p.off_vals_copy;

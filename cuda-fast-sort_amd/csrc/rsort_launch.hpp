@@ -2,7 +2,9 @@
 // onesweep_byte_sort_enqueue, include/rsort/detail/radix_kernel.cuh:336-474):
 //   clear control block -> histogram (+ lookback clear, offsets, skip mask)
 //   -> one onesweep launch per pass -> device-decided copy-back.
-// Every launch goes on the caller's stream; nothing here allocates.
+// Every launch of a plain sort goes on the caller's stream; the segmented
+// sort forks onto the library's side streams after its classification and
+// joins back before it returns.  Nothing here allocates.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -79,6 +81,13 @@ struct SegJob {
     hipEvent_t* events;
     int max_events;
     int* n_events;
+    // Side streams (null: one stream): the LDS bins and the long segments
+    // sort disjoint segments, so after the classification they run on the
+    // caller's stream and three side streams at once, forked and joined with
+    // these events (a graph capture records the same fork and join).
+    hipStream_t side[3];
+    hipEvent_t fork;
+    hipEvent_t join[3];
 };
 
 // Requests sorted in place by the LDS bins, one workgroup each (the serving

@@ -56,13 +56,12 @@ hipError_t launch_req(const ReqJob& j, hipStream_t s) {
 }
 
 template <int KB, int VB, bool WIDE>
+hipError_t launch_seg_long(SegParams a, const SegJob& j, hipStream_t s);
+
+template <int KB, int VB, bool WIDE>
 hipError_t launch_seg_t(const SegJob& j, hipStream_t s) {
-    using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
-    using KM = typename KeyTraits<KB>::Mem;
-    using VM = typename ValTraits<VB>::Mem;
     constexpr TileCfg C = tile_cfg(KB, VB);
     constexpr uint32_t P = KB;
-    (void)sizeof(Off);
 
     SegParams a{};
     a.keys = j.keys;
@@ -98,16 +97,48 @@ hipError_t launch_seg_t(const SegJob& j, hipStream_t s) {
     hipLaunchKernelGGL(k_seg_fill, dim3(j.chunks), dim3(kSegChunk), 0, s, a);
     record_event(j, s);
 
-    // 2. LDS bins, longest segments first
-    hipError_t e = launch_seg_bin<KB, VB, 4>(a, j, s);
-    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 3>(a, j, s);
-    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 2>(a, j, s);
-    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 1>(a, j, s);
+    // 2. LDS bins and long segments; with side streams they (disjoint
+    //    segments) run at once, so that one kernel's tail and spare CU
+    //    resources are filled by another's workgroups
+    const bool fork = j.side[0] && !j.events;
+    hipStream_t sb[3] = {s, s, s};  // bin 3 | bin 2 | bin 1, then the long segments
+    hipError_t e = hipSuccess;
+    if (fork) {
+        e = hipEventRecord(j.fork, s);
+        for (int i = 0; i < 3 && e == hipSuccess; ++i) {
+            e = hipStreamWaitEvent(j.side[i], j.fork, 0);
+            sb[i] = j.side[i];
+        }
+        if (e != hipSuccess) return e;
+    }
+    // Measured on MI355X (100k-segment u32/u32 batch): with the bins on one
+    // stream 92.0 GKeys/s; bins 4,3 | 2 | 1,0 | long 94.2; 4,1 | 3 | 2,0 |
+    // long 96.9; 3,0 | 2 | 4,1 | long 100.2; this assignment 100.3 (the two
+    // largest bins, 2 and 3, start at once on separate streams)
+    e = launch_seg_bin<KB, VB, 4>(a, j, s);
     if (e == hipSuccess) e = launch_seg_bin<KB, VB, 0>(a, j, s);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 3>(a, j, sb[0]);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 2>(a, j, sb[1]);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 1>(a, j, sb[2]);
     if (e != hipSuccess) return e;
     record_event(j, s);
+    e = launch_seg_long<KB, VB, WIDE>(a, j, sb[2]);
+    if (fork) {
+        for (int i = 0; i < 3 && e == hipSuccess; ++i) {
+            e = hipEventRecord(j.join[i], j.side[i]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s, j.join[i], 0);
+        }
+    }
+    return e;
+}
 
-    // 3. long segments: tile table, digit offsets, onesweep passes
+// 3. long segments: tile table, digit offsets, onesweep passes
+template <int KB, int VB, bool WIDE>
+hipError_t launch_seg_long(SegParams a, const SegJob& j, hipStream_t s) {
+    using KM = typename KeyTraits<KB>::Mem;
+    using VM = typename ValTraits<VB>::Mem;
+    constexpr TileCfg C = tile_cfg(KB, VB);
+    constexpr uint32_t P = KB;
     if (j.max_large == 0) return hipGetLastError();
     const uint32_t gl = grid_for(j.max_large, 1, j.num_cus, 4);
     hipLaunchKernelGGL((k_seg_large_init<WIDE>), dim3(gl), dim3(256), 0, s, a);

@@ -291,6 +291,42 @@ struct Server {
         return e;
     }
 
+    // Runs every kernel the direct form can use once, on the slot's own
+    // buffers (their contents are not data yet): one request per LDS bin, and
+    // rsort_sort at sizes from just above the LDS threshold up to batch_keys
+    // by factors of 4, which covers the tuned tile classes.  A kernel's
+    // first launch in a process loads its code object, which stalled the
+    // first long request of a served stream by ~16 ms (p99 8 ms over 5000
+    // Poisson requests) before this warm-up.
+    hipError_t warm_direct(Slot& sl) {
+        sl.items.clear();
+        uint64_t at = 0, longest = 0;
+        for (int b = 0; b < kSegBins; ++b) {
+            if (!cap[b]) continue;
+            longest = cap[b];
+            const uint64_t n = cap[b] < cfg.batch_keys - at ? cap[b] : cfg.batch_keys - at;
+            if (n < 2) break;
+            SegReq r{};
+            r.keys = (uint64_t)(uintptr_t)((uint8_t*)sl.bkeys + at * kb);
+            r.vals = vb ? (uint64_t)(uintptr_t)((uint8_t*)sl.bvals + at * vb) : 0u;
+            r.n = n;
+            sl.items.push_back(r);
+            at += n;
+        }
+        hipError_t e = enqueue_direct(sl);
+        uint64_t n = longest + 1u > 2u ? longest + 1u : 2u;
+        while (n <= cfg.batch_keys && e == hipSuccess) {
+            size_t tb = sl.flat_ws_bytes;
+            if (rsort_sort(&cfg.desc, sl.bkeys, sl.bvals, sl.flat_ws, &tb, n, sl.stream) != RSORT_SUCCESS)
+                e = hipErrorLaunchFailure;
+            if (n == cfg.batch_keys) break;
+            n = n > cfg.batch_keys / 4u ? cfg.batch_keys : n * 4u;
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(sl.stream);
+        sl.items.clear();
+        return e;
+    }
+
     void fail_with(rsort_status st, const char* what, hipError_t e) {
         std::lock_guard<std::mutex> lk(mu);
         if (error == RSORT_SUCCESS) {
@@ -516,6 +552,7 @@ rsort_status rsort_serve_create(const rsort_serve_config* cfg, rsort_server** ou
             if (e == hipSuccess) e = hipMalloc(&sl.flat_ws, flat_need);
         }
     }
+    if (e == hipSuccess && s->direct) e = s->warm_direct(s->slots[0]);
     // one direct run per slot (empty batch: also initialises the launchers'
     // one-time queries), then capture the batch into a graph
     for (uint32_t i = 0; i < cfg->depth && e == hipSuccess && !s->direct; ++i) {

@@ -148,9 +148,33 @@ def test_offsets_clamped_and_gaps(rs, orc):
     n = 50000
     keys = orc.mt19937(5, n)
     vals = np.arange(n, dtype=np.uint32)
-    # gap [100, 900), empty segments, a decreasing pair, an end past n
-    off = np.array([0, 100, 900, 900, 5000, 4000, 4000, 30000, 10**9], dtype=np.uint64)
+    # disjoint segments out of order: [30000, n) (an end past n, a long
+    # segment), [100, 900), [900, 5000); decreasing pairs (10^9 -> 100,
+    # 5000 -> 4000) and empty segments between them; keys [0, 100) and
+    # [5000, 30000) lie in no segment and stay as they are
+    off = np.array([30000, 10**9, 100, 900, 900, 5000, 4000, 4000], dtype=np.uint64)
     check(rs, orc, keys, vals, off, 0, 4, 4, False)
+
+
+def test_overlapping_segments_stay_in_bounds(rs, orc):
+    """Overlapping segments sort in an unspecified order (the bins and the
+    long segments run at once): the segments that overlap no other are
+    exact, and nothing outside the segments' union changes."""
+    n = 60000
+    keys = orc.mt19937(6, n)
+    vals = np.arange(n, dtype=np.uint32)
+    # [900, 5000) (an LDS bin) overlaps [4000, 30000) (long); [0, 100),
+    # [100, 900) and [30000, 50000) overlap nothing; [50000, 60000) is in no
+    # segment
+    off = np.array([0, 100, 900, 900, 5000, 4000, 4000, 30000, 50000], dtype=np.uint64)
+    kb = np.ascontiguousarray(keys).view(np.uint8).reshape(-1)
+    gk, gv = gpu_segmented(rs, kb, vals.view(np.uint8), n, off, 0, 4, 4, False)
+    gk, gv = gk.view(np.uint32), gv.view(np.uint32)
+    for lo, hi in ((0, 100), (100, 900), (30000, 50000)):
+        ek, ev = orc.sort(keys[lo:hi].view(np.uint8), vals[lo:hi], 0, 4, False, vb=4)
+        assert np.array_equal(gk[lo:hi], ek.view(np.uint32)), f"segment [{lo}, {hi}) keys"
+        assert np.array_equal(gv[lo:hi], ev), f"segment [{lo}, {hi}) values"
+    assert np.array_equal(gk[50000:], keys[50000:]) and np.array_equal(gv[50000:], vals[50000:])
 
 
 def test_workspace_reuse_and_tiny_segments(rs, orc):
