@@ -559,7 +559,7 @@ def bench_segments(rs, a, steps, warmup):
     seg_of = torch.repeat_interleave(torch.arange(a.segments, dtype=torch.int32, device=dev),
                                      torch.from_numpy(sizes).to(dev))
     ok = ok and bool((seg_of[v.long()] == seg_of).all().item())
-    lds_cap = rs.segmented_plan(desc, n, 8, a.segments).bin_capacity[4]
+    lds_cap = rs.segmented_plan(desc, n, 8, a.segments).bin_capacity[-1]
     small = int(sizes[sizes <= lds_cap].sum())
     big = n - small
     bytes_ = 2.0 * small * 8 + 5 * 2.0 * big * 8  # LDS segments: one read + one write; long: the formula

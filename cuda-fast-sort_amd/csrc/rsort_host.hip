@@ -340,7 +340,7 @@ rsort_status make_seg_plan(const rsort_desc* d, uint64_t n, uint32_t offset_byte
         p->off_vals_copy = reserve(&off, (size_t)n * p->vb_user);
     }
     p->off_ctrl = reserve(&off, sizeof(SegCtrl));
-    p->off_chunk = reserve(&off, (size_t)p->chunks * 8u * 4u);
+    p->off_chunk = reserve(&off, (size_t)p->chunks * kSegRow * 4u);
     p->off_queue = reserve(&off, (size_t)num_segments * 4u);
     p->off_desc = reserve(&off, (size_t)p->max_large * 32u);
     p->off_tile_seg = reserve(&off, (size_t)p->max_tiles * 4u);

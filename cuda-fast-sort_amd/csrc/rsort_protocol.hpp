@@ -52,18 +52,18 @@ RSA_HD uint32_t pass_run_mask(uint32_t skip_mask, uint32_t passes) {
     return active;
 }
 
-// Control block of the segmented sort (same 256-byte slot, cleared per sort).
-// Categories 0..4 are the LDS bins, 5 the onesweep segments.
+// Control block of the segmented sort (512 bytes, cleared per sort).
+// Categories 0..kSegBins-1 are the LDS bins, kSegBins the onesweep segments.
 struct SegCtrl {
     uint32_t ticket[kMaxPasses];  // onesweep pass tickets over the long segments' tiles
-    uint32_t bin_ticket[8];       // work tickets of the LDS bins
-    uint32_t cat_count[8];        // segments per category
-    uint32_t cat_base[8];         // first queue slot of each category
+    uint32_t bin_ticket[16];      // work tickets of the LDS bins
+    uint32_t cat_count[16];       // segments per category
+    uint32_t cat_base[16];        // first queue slot of each category
     uint32_t large_tiles;         // onesweep tiles of all long segments
     uint32_t overflow;            // 1: segments overlap beyond the workspace bounds
-    uint32_t reserved[22];
+    uint32_t reserved[62];
 };
-static_assert(sizeof(SegCtrl) == 256, "segmented control header must stay 256 bytes");
+static_assert(sizeof(SegCtrl) == 512, "segmented control header must stay 512 bytes");
 
 // One request of the direct (pointer-list) form of the segmented LDS bins:
 // its own key and value arrays, sorted in place.

@@ -34,11 +34,16 @@ hipError_t launch_req_v(const ReqJob& j, hipStream_t s) {
     SegParams a{};
     a.reqs = (const SegReq*)j.reqs;
     a.tw = j.tw;
+    static_assert(kSegBins == 9, "one launch per bin below");
     launch_req_bin<KB, VB, 0>(a, j.first, j.count, s);
     launch_req_bin<KB, VB, 1>(a, j.first, j.count, s);
     launch_req_bin<KB, VB, 2>(a, j.first, j.count, s);
     launch_req_bin<KB, VB, 3>(a, j.first, j.count, s);
     launch_req_bin<KB, VB, 4>(a, j.first, j.count, s);
+    launch_req_bin<KB, VB, 5>(a, j.first, j.count, s);
+    launch_req_bin<KB, VB, 6>(a, j.first, j.count, s);
+    launch_req_bin<KB, VB, 7>(a, j.first, j.count, s);
+    launch_req_bin<KB, VB, 8>(a, j.first, j.count, s);
     return hipGetLastError();
 }
 
@@ -93,7 +98,7 @@ hipError_t launch_seg_t(const SegJob& j, hipStream_t s) {
     record_event(j, s);
     hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(64), 0, s, (uint32_t*)j.ctrl, (uint32_t)(sizeof(SegCtrl) / 4u));
     hipLaunchKernelGGL(k_seg_count, dim3(j.chunks), dim3(kSegChunk), 0, s, a);
-    hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(512), 0, s, a, j.chunks);
+    hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kSegScanThreads), 0, s, a, j.chunks);
     hipLaunchKernelGGL(k_seg_fill, dim3(j.chunks), dim3(kSegChunk), 0, s, a);
     record_event(j, s);
 
@@ -101,7 +106,7 @@ hipError_t launch_seg_t(const SegJob& j, hipStream_t s) {
     //    segments) run at once, so that one kernel's tail and spare CU
     //    resources are filled by another's workgroups
     const bool fork = j.side[0] && !j.events;
-    hipStream_t sb[3] = {s, s, s};  // bin 3 | bin 2 | bin 1, then the long segments
+    hipStream_t sb[3] = {s, s, s};
     hipError_t e = hipSuccess;
     if (fork) {
         e = hipEventRecord(j.fork, s);
@@ -111,18 +116,28 @@ hipError_t launch_seg_t(const SegJob& j, hipStream_t s) {
         }
         if (e != hipSuccess) return e;
     }
-    // Measured on MI355X (100k-segment u32/u32 batch): with the bins on one
-    // stream 92.0 GKeys/s; bins 4,3 | 2 | 1,0 | long 94.2; 4,1 | 3 | 2,0 |
-    // long 96.9; 3,0 | 2 | 4,1 | long 100.2; this assignment 100.3 (the two
-    // largest bins, 2 and 3, start at once on separate streams)
-    e = launch_seg_bin<KB, VB, 4>(a, j, s);
-    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 0>(a, j, s);
-    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 3>(a, j, sb[0]);
-    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 2>(a, j, sb[1]);
-    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 1>(a, j, sb[2]);
-    if (e != hipSuccess) return e;
+    // Four streams, the longest-running bins first on separate streams, the
+    // long segments behind the short bin 7 (measured spans when run side by
+    // side: bin 5 ~2.0 ms, bin 8 ~1.1 ms, bin 4 ~0.9 ms, bins 0-3, 6 and 7
+    // 0.25-0.6 ms, the long segments ~0.5 ms).  Measured on MI355X, 100k
+    // lognormal segments, u32/u32 (GKeys/s): nine bins with 3,0 | 5,8 | 4,2
+    // | 7,6,1,long 100.4; 7 | 5,long | 4,3,1 | 0,2,8,6 102.3, 103.3 with the
+    // large bins exchanging keys and values separately; this one 105.0.
+    // (Five bins of capacity 1024 << b: one stream 92.0; 4,3 | 2 | 1,0 |
+    // long 94.2; 4,1 | 3 | 2,0 | long 96.9; 4,0 | 3 | 2 | 1,long 100.3; the
+    // long segments first on their own stream 95-100.)
+    static_assert(kSegBins == 9, "one launch per bin below");
+    e = launch_seg_bin<KB, VB, 7>(a, j, s);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 5>(a, j, sb[0]);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 4>(a, j, sb[1]);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 0>(a, j, sb[2]);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 3>(a, j, sb[1]);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 2>(a, j, sb[2]);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 8>(a, j, sb[2]);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 1>(a, j, sb[1]);
     record_event(j, s);
-    e = launch_seg_long<KB, VB, WIDE>(a, j, sb[2]);
+    if (e == hipSuccess) e = launch_seg_long<KB, VB, WIDE>(a, j, s);
+    if (e == hipSuccess) e = launch_seg_bin<KB, VB, 6>(a, j, s);
     if (fork) {
         for (int i = 0; i < 3 && e == hipSuccess; ++i) {
             e = hipEventRecord(j.join[i], j.side[i]);

@@ -4,8 +4,8 @@
 // sort (the same stable LSD order inside each segment).
 //
 //   k_seg_count / k_seg_scan / k_seg_fill
-//       classify every segment by length into an LDS bin (0..4) or the
-//       onesweep category (5), and build per-category queues of segment ids
+//       classify every segment by length into an LDS bin (0..kSegBins-1) or
+//       the onesweep category (kSegBins), and build per-category queues of segment ids
 //       (in segment order) plus a descriptor and a tile range per long
 //       segment -- three launches, no host round trip.
 //   k_seg_lds<bin>
@@ -38,7 +38,7 @@ struct SegParams {
     void* tmp_vals;
     const void* offsets;   // num_segments + 1 offsets, 32- or 64-bit
     SegCtrl* ctrl;
-    uint32_t* chunk;       // [chunks][8] category counts, then exclusive bases
+    uint32_t* chunk;       // [chunks][kSegRow] category counts, then exclusive bases
     uint32_t* queue;       // [num_segments] segment ids grouped by category
     SegDesc* desc;         // [max_large] long segments
     uint32_t* tile_seg;    // [max_tiles] long-segment index of each tile
@@ -84,13 +84,14 @@ __device__ __forceinline__ uint32_t seg_category(const SegParams& a, uint64_t m)
 }
 
 // ---------------------------------------------------------------------------
-// Classification.  chunk[c*8 + k]: k = 0..5 segments of category k in chunk
-// c, k = 6 onesweep tiles of chunk c's long segments.
+// Classification.  chunk[c*kSegRow + k]: k = 0..kSegCatLarge segments of
+// category k in chunk c, k = kSegCatTiles onesweep tiles of chunk c's long
+// segments.
 // ---------------------------------------------------------------------------
 static __global__ __launch_bounds__(kSegChunk) void k_seg_count(SegParams a) {
-    __shared__ uint32_t s_cnt[8];
+    __shared__ uint32_t s_cnt[kSegRow];
     const uint32_t tid = threadIdx.x;
-    if (tid < 8) s_cnt[tid] = 0u;
+    if (tid < kSegRow) s_cnt[tid] = 0u;
     __syncthreads();
     const uint64_t s = (uint64_t)blockIdx.x * kSegChunk + tid;
     uint32_t cat = kSegCatNone, tiles = 0;
@@ -108,24 +109,25 @@ static __global__ __launch_bounds__(kSegChunk) void k_seg_count(SegParams a) {
     uint32_t t = tiles;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-    if ((tid & 63u) == 0u && t) atomicAdd(&s_cnt[6], t);
+    if ((tid & 63u) == 0u && t) atomicAdd(&s_cnt[kSegCatTiles], t);
     __syncthreads();
-    if (tid < 8) a.chunk[blockIdx.x * 8u + tid] = s_cnt[tid];
+    if (tid < kSegRow) a.chunk[blockIdx.x * kSegRow + tid] = s_cnt[tid];
 }
 
 // One workgroup: wave k scans column k of the chunk table (exclusive), then
 // the category totals become the queue layout.
-static __global__ __launch_bounds__(512) void k_seg_scan(SegParams a, uint32_t chunks) {
-    __shared__ uint32_t s_tot[8];
+constexpr uint32_t kSegScanThreads = 64u * (kSegCatTiles + 1u);
+static __global__ __launch_bounds__(kSegScanThreads) void k_seg_scan(SegParams a, uint32_t chunks) {
+    __shared__ uint32_t s_tot[kSegRow];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t c = threadIdx.x >> 6;
-    if (c < 7u) {
+    if (c <= kSegCatTiles) {
         uint32_t carry = 0;
         for (uint32_t k0 = 0; k0 < chunks; k0 += 64u) {
             const uint32_t k = k0 + lane;
-            const uint32_t v = k < chunks ? a.chunk[k * 8u + c] : 0u;
+            const uint32_t v = k < chunks ? a.chunk[k * kSegRow + c] : 0u;
             const uint32_t incl = wave_incl_scan<uint32_t>(v);
-            if (k < chunks) a.chunk[k * 8u + c] = carry + incl - v;
+            if (k < chunks) a.chunk[k * kSegRow + c] = carry + incl - v;
             carry += __shfl(incl, 63);
         }
         if (lane == 0) s_tot[c] = carry;
@@ -138,11 +140,11 @@ static __global__ __launch_bounds__(512) void k_seg_scan(SegParams a, uint32_t c
             a.ctrl->cat_base[cc] = base;
             base += s_tot[cc];
         }
-        a.ctrl->large_tiles = s_tot[6];
+        a.ctrl->large_tiles = s_tot[kSegCatTiles];
         // Disjoint segments never exceed the bounds the workspace was sized
         // for; overlapping ones can: then the long segments are left unsorted
         // (flagged) rather than written past the tables.
-        if (s_tot[kSegCatLarge] > a.max_large || s_tot[6] > a.max_tiles) {
+        if (s_tot[kSegCatLarge] > a.max_large || s_tot[kSegCatTiles] > a.max_tiles) {
             a.ctrl->overflow = 1u;
             a.ctrl->cat_count[kSegCatLarge] = 0u;
             a.ctrl->large_tiles = 0u;
@@ -154,7 +156,7 @@ static __global__ __launch_bounds__(512) void k_seg_scan(SegParams a, uint32_t c
 // segments' descriptors.
 static __global__ __launch_bounds__(kSegChunk) void k_seg_fill(SegParams a) {
     constexpr int WAVES = kSegChunk / 64;
-    __shared__ uint32_t s_wc[WAVES][8];
+    __shared__ uint32_t s_wc[WAVES][kSegRow];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t w = tid >> 6;
@@ -174,21 +176,21 @@ static __global__ __launch_bounds__(kSegChunk) void k_seg_fill(SegParams a) {
         if (lane == 0) s_wc[w][c] = (uint32_t)__builtin_popcountll(m);
     }
     const uint32_t tincl = wave_incl_scan<uint32_t>(tiles);
-    if (lane == 63u) s_wc[w][6] = tincl;
+    if (lane == 63u) s_wc[w][kSegCatTiles] = tincl;
     __syncthreads();
     if (cat <= kSegCatLarge) {
         uint32_t pre = 0, tpre = 0;
         for (uint32_t ww = 0; ww < w; ++ww) {
             pre += s_wc[ww][cat];
-            tpre += s_wc[ww][6];
+            tpre += s_wc[ww][kSegCatTiles];
         }
-        const uint32_t pos = a.chunk[blockIdx.x * 8u + cat] + pre + rank;
+        const uint32_t pos = a.chunk[blockIdx.x * kSegRow + cat] + pre + rank;
         a.queue[a.ctrl->cat_base[cat] + pos] = (uint32_t)s;
         if (cat == kSegCatLarge && pos < a.max_large) {
             SegDesc d;
             d.begin = b;
             d.end = e;
-            d.first_tile = a.chunk[blockIdx.x * 8u + 6u] + tpre + tincl - tiles;
+            d.first_tile = a.chunk[blockIdx.x * kSegRow + kSegCatTiles] + tpre + tincl - tiles;
             d.tiles = tiles;
             d.seg = s;
             a.desc[pos] = d;
@@ -215,8 +217,15 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
 #define RSA_SEG_PAIR_XCHG 1
 #endif
     // 4-byte keys with 4-byte values: one 64-bit exchange moves both, with
-    // half the exchange instructions and barriers (twice the staging LDS)
-    constexpr bool PAIR = RSA_SEG_PAIR_XCHG && KB == 4 && VB == 4;
+    // half the exchange instructions and barriers (twice the staging LDS).
+    // Bins of more threads keep keys and values apart: their staging LDS then
+    // leaves room for other bins' workgroups on the CU (measured on MI355X,
+    // the 100k-segment batch: paired up to 1024 threads 102.3 GKeys/s, up to
+    // 384 103.3-104.2, up to 256 104.5-105.6).
+#ifndef RSA_SEG_PAIR_MAX_THREADS
+#define RSA_SEG_PAIR_MAX_THREADS 256
+#endif
+    constexpr bool PAIR = RSA_SEG_PAIR_XCHG && KB == 4 && VB == 4 && THREADS <= RSA_SEG_PAIR_MAX_THREADS;
     constexpr size_t BIG0 = KSTAGE > VSTAGE ? (KSTAGE > MATCH ? KSTAGE : MATCH) : (VSTAGE > MATCH ? VSTAGE : MATCH);
     constexpr size_t BIG = PAIR ? (KSTAGE + VSTAGE > MATCH ? KSTAGE + VSTAGE : MATCH) : BIG0;
     // digits owned per thread in the digit scan, and how many threads own digits

@@ -176,28 +176,41 @@ RSA_HD constexpr TileCfg tile_cand(int key_bytes, int val_bytes, int cand) {
 
 // ---------------------------------------------------------------------------
 // Segmented sort.  A segment of at most seg_cap(kb, vb, kSegBins-1) keys is
-// sorted by one workgroup entirely in LDS; bin b uses 64 << b threads holding
-// seg_items(kb, vb) keys each, and takes the segments too long for bin b-1.
-// Longer segments go through the onesweep passes with per-segment digit
-// offsets.  The item count keeps one bin's keys (or values) at <= 64 KiB of LDS.
+// sorted by one workgroup entirely in LDS; bin b takes the segments too long
+// for bin b-1.  Longer segments go through the onesweep passes with
+// per-segment digit offsets.  The item count keeps one bin's keys (or values)
+// at <= 64 KiB of LDS.
 // ---------------------------------------------------------------------------
-constexpr int kSegBins = 5;
-constexpr uint32_t kSegCatLarge = kSegBins;  // category of the onesweep segments
-constexpr uint32_t kSegCatNone = 7;          // 0 or 1 key: nothing to do
-constexpr uint32_t kSegChunk = 1024;         // segments per classification workgroup
+constexpr int kSegBins = 9;
+constexpr uint32_t kSegCatLarge = kSegBins;      // category of the onesweep segments
+constexpr uint32_t kSegCatTiles = kSegBins + 1;  // chunk-table column: the long segments' tiles
+constexpr uint32_t kSegRow = 16;                 // chunk-table columns per classification chunk
+constexpr uint32_t kSegCatNone = kSegRow - 1;    // 0 or 1 key: nothing to do
+constexpr uint32_t kSegChunk = 1024;             // segments per classification workgroup
+static_assert(kSegCatTiles < kSegCatNone, "categories and the tile column fit one chunk row");
 
 RSA_HD constexpr uint32_t seg_items(int key_bytes, int val_bytes) {
     const int wide = key_bytes > val_bytes ? key_bytes : val_bytes;
     return wide <= 4 ? 16u : (wide == 8 ? 8u : 4u);
 }
 
-// Bin b: threads and items per thread.  Bin 0 runs two waves at half the
-// items: its one-wave segments were bound by the latency of each wave's rank
-// chain.  (Measured on MI355X: twice the threads at half the items in the
-// larger bins, which are bound by LDS throughput, was slower.)
-RSA_HD constexpr uint32_t seg_bin_threads(int, int, int bin) { return bin == 0 ? 128u : 64u << bin; }
+// Bin b: threads, and items per thread as a fraction of seg_items (in
+// quarters).  Capacities grow by 1.33x or 1.5x from bin to bin (1024, 1536,
+// 2048, 3072, 4096, 6144, 8192, 12288, 16384 keys for 4-byte keys and
+// values), so a segment fills at least 2/3 of its workgroup's slots: the
+// bins are bound by LDS throughput, and filler slots cost LDS work too.
+// Measured on MI355X (100k lognormal segments, u32/u32): five bins of
+// capacity 1024 << b, ~70 % full, 100.3 GKeys/s; these nine, see the design
+// notes.  Bin 0 runs two waves at half the items: its one-wave segments were
+// bound by the latency of each wave's rank chain.  (Twice the threads at half
+// the items in the larger bins, which are bound by LDS throughput, was
+// slower.)
+RSA_HD constexpr uint32_t seg_bin_threads(int, int, int bin) {
+    return bin <= 2 ? 128u : bin <= 4 ? 256u : bin == 5 ? 384u : bin == 6 ? 512u : bin == 7 ? 768u : 1024u;
+}
+RSA_HD constexpr uint32_t seg_bin_quarters(int bin) { return bin == 0 ? 2u : (bin == 1 || bin == 3) ? 3u : 4u; }
 RSA_HD constexpr uint32_t seg_bin_items(int key_bytes, int val_bytes, int bin) {
-    return bin == 0 ? seg_items(key_bytes, val_bytes) / 2u : seg_items(key_bytes, val_bytes);
+    return seg_items(key_bytes, val_bytes) * seg_bin_quarters(bin) / 4u;
 }
 
 RSA_HD constexpr uint32_t seg_cap(int key_bytes, int val_bytes, int bin) {

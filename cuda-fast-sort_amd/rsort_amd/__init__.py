@@ -67,9 +67,12 @@ class PlanInfo(ctypes.Structure):
     ]
 
 
+SEGMENT_BINS = 9  # RSORT_SEGMENT_BINS
+
+
 class SegPlanInfo(ctypes.Structure):
     _fields_ = [
-        ("bin_capacity", ctypes.c_uint32 * 5),
+        ("bin_capacity", ctypes.c_uint32 * SEGMENT_BINS),
         ("tile", ctypes.c_uint32),
         ("max_long_segments", ctypes.c_uint64),
         ("max_tiles", ctypes.c_uint64),

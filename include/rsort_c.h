@@ -181,7 +181,7 @@ rsort_status rsort_plan(const rsort_desc* desc, uint64_t n, rsort_plan_info* out
  *   n            extent of keys / values in elements
  *   workspace    as rsort_sort: NULL -> size query into *temp_bytes
  *
- * A segment of at most bin_capacity[4] keys (rsort_segmented_plan; 16384 for
+ * A segment of at most bin_capacity[8] keys (rsort_segmented_plan; 16384 for
  * 4-byte keys and values) is sorted by one workgroup in LDS, longer ones by
  * the onesweep passes.  Stream-ordered, no host synchronisation, so the whole
  * call can be captured into a hipGraph.
@@ -190,12 +190,12 @@ rsort_status rsort_segmented_sort(const rsort_desc* desc, void* keys, void* valu
                                   uint32_t offset_bytes, uint64_t num_segments, void* workspace, size_t* temp_bytes,
                                   void* stream);
 
-#define RSORT_SEGMENT_BINS 5
+#define RSORT_SEGMENT_BINS 9
 
 typedef struct rsort_segmented_plan_info {
-    uint32_t bin_capacity[RSORT_SEGMENT_BINS]; /* keys per LDS workgroup of 64<<b threads */
+    uint32_t bin_capacity[RSORT_SEGMENT_BINS]; /* keys per LDS workgroup of bin b (ascending) */
     uint32_t tile;                /* onesweep tile of the long segments */
-    uint64_t max_long_segments;   /* workspace bound on segments longer than bin_capacity[4] */
+    uint64_t max_long_segments;   /* workspace bound on segments longer than bin_capacity[8] */
     uint64_t max_tiles;
     uint32_t wide;
     uint32_t index_mode;
@@ -249,7 +249,7 @@ rsort_status rsort_digit_counts(const rsort_desc* desc, const void* keys, uint64
  *                       elsewhere they mean 0)
  *   hist_blocks_per_cu  histogram workgroups per CU, 1..8
  *   lds_threshold       segmented sort: longest segment sorted in LDS (LDS
- *                       bins larger than it are not used), <= bin_capacity[4]
+ *                       bins larger than it are not used), <= bin_capacity[8]
  *   small_tile_candidate, small_log2n
  *                       the tile candidate of sorts below 2^small_log2n keys
  *                       (few tiles: a smaller tile fills more CUs)

@@ -139,7 +139,8 @@ def test_segmented_plan_bins(rs, kb, vb, cap):
     d = rs.make_desc(0, kb, vb)
     p = rs.segmented_plan(d, 1 << 22, 4, 1000)
     caps = list(p.bin_capacity)
-    assert caps[-1] == cap and caps == [cap >> (4 - b) for b in range(5)]
+    # nine bins: 1/16, 3/32, 1/8, 3/16, 1/4, 3/8, 1/2, 3/4 and all of the largest
+    assert caps[-1] == cap and caps == [cap * q // 32 for q in (2, 3, 4, 6, 8, 12, 16, 24, 32)]
     # at most n / (cap + 1) segments can be too long for the LDS bins
     assert p.max_long_segments == min(1000, (1 << 22) // (cap + 1))
     assert p.max_tiles == ((1 << 22) + p.tile - 1) // p.tile + p.max_long_segments

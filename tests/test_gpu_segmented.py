@@ -61,8 +61,9 @@ def check(rs, orc, keys, vals, offsets, kind, kb, vb, desc, ws=None):
 
 # segment lengths around every LDS bin edge (4-byte keys: 1K, 2K, 4K, 8K, 16K)
 # and into the onesweep path
-EDGE_LENGTHS = [0, 1, 2, 3, 63, 64, 65, 1023, 1024, 1025, 2047, 2048, 2049, 4095, 4096, 4097, 8191, 8192, 8193,
-                16383, 16384, 16385, 40000, 131072 + 17]
+EDGE_LENGTHS = [0, 1, 2, 3, 63, 64, 65, 1023, 1024, 1025, 1535, 1536, 1537, 2047, 2048, 2049, 3071, 3072, 3073,
+                4095, 4096, 4097, 6143, 6144, 6145, 8191, 8192, 8193, 12287, 12288, 12289, 16383, 16384, 16385,
+                40000, 131072 + 17]
 
 
 @pytest.mark.parametrize("desc", [False, True])
