@@ -714,12 +714,16 @@ rsort_status run_sort_noalloc(const rsort_desc* d, void* keys, void* values, voi
     return enqueue_sort(p, keys, values, b, n, stream, nullptr, nullptr, 0, nullptr);
 }
 
-// Side streams of the segmented sort, per device: one set for direct calls
-// (the fork and join are enqueued under a per-device lock, so that calls from
-// several host threads cannot interleave the events' records and waits) and
-// one for graph captures (used only under the graph cache's lock).  Created
-// once, non-blocking, at the default priority; kept until the process ends.
-// RSORT_AMD_SEG_STREAMS=0 keeps every launch on the caller's stream.
+// Side streams of the segmented sort: per device, one set for direct calls,
+// whose fork and join are enqueued under a per-device lock (so that calls
+// from several host threads cannot interleave the events' records and
+// waits); and, per host thread and device, one set for calls made while the
+// caller's stream is being captured (the library's own graph capture, or a
+// caller's): a side stream joined into a capture stays in it until that
+// capture ends, so two threads capturing at once must not share one.
+// Created on first use, non-blocking, at the default priority; kept until
+// the process ends.  RSORT_AMD_SEG_STREAMS=0 keeps every launch on the
+// caller's stream.
 struct SideSet {
     hipStream_t s[3];
     hipEvent_t fork, join[3];
@@ -734,22 +738,40 @@ bool seg_streams_allowed() {
     return v;
 }
 
+void side_set_init(SideSet& ss) {
+    hipError_t e = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming);
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipStreamCreateWithFlags(&ss.s[i], hipStreamNonBlocking);
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&ss.join[i], hipEventDisableTiming);
+    ss.ok = e == hipSuccess;
+    (void)hipGetLastError();
+}
+
 SideSet* side_set(int dev, bool capture) {
-    static std::mutex mu;
-    static SideSet sets[64][2];
-    static bool done[64][2] = {{false}};
     if (dev < 0 || dev >= 64 || !seg_streams_allowed()) return nullptr;
-    std::lock_guard<std::mutex> lk(mu);
-    SideSet& ss = sets[dev][capture ? 1 : 0];
-    if (!done[dev][capture ? 1 : 0]) {
-        done[dev][capture ? 1 : 0] = true;
-        hipError_t e = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming);
-        for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipStreamCreateWithFlags(&ss.s[i], hipStreamNonBlocking);
-        for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&ss.join[i], hipEventDisableTiming);
-        ss.ok = e == hipSuccess;
-        (void)hipGetLastError();
+    if (capture) {
+        thread_local SideSet tsets[64];
+        thread_local bool tdone[64] = {false};
+        if (!tdone[dev]) {
+            tdone[dev] = true;
+            // first use inside a capture: creating streams and events is not
+            // capturable work, so this thread's capture mode is relaxed for it
+            hipStreamCaptureMode mode = hipStreamCaptureModeRelaxed;
+            const bool swapped = hipThreadExchangeStreamCaptureMode(&mode) == hipSuccess;
+            side_set_init(tsets[dev]);
+            if (swapped) (void)hipThreadExchangeStreamCaptureMode(&mode);
+            (void)hipGetLastError();
+        }
+        return tsets[dev].ok ? &tsets[dev] : nullptr;
     }
-    return ss.ok ? &ss : nullptr;
+    static std::mutex mu;
+    static SideSet sets[64];
+    static bool done[64] = {false};
+    std::lock_guard<std::mutex> lk(mu);
+    if (!done[dev]) {
+        done[dev] = true;
+        side_set_init(sets[dev]);
+    }
+    return sets[dev].ok ? &sets[dev] : nullptr;
 }
 
 std::mutex& side_lock(int dev) {
@@ -844,7 +866,7 @@ rsort_status run_segmented(const rsort_desc* d, void* keys, void* values, uint64
                 js.join[i] = ss->join[i];
             }
             js.fork = ss->fork;
-            // direct calls share the set's events across host threads
+            // the direct set's streams and events are shared by every host thread
             std::unique_lock<std::mutex> lk(side_lock(dev), std::defer_lock);
             if (cap == hipStreamCaptureStatusNone) lk.lock();
             e2 = dispatch_seg(js, s);

@@ -109,3 +109,81 @@ def test_graph_cache_many_argument_sets(rs, orc):
         torch.cuda.synchronize()
         for i, b in enumerate(bufs):
             assert np.array_equal(b.cpu().numpy(), ek), f"round {rnd} buffer {i}"
+
+
+def test_segmented_in_callers_capture(rs, orc):
+    """A caller captures a segmented sort (no RSORT_FLAG_GRAPH) into its own
+    graph: the fork onto the library's side streams and the join back become
+    branches of that graph.  Every replay, with eager refills and
+    synchronisations between them, equals the oracle."""
+    lengths = [int(x) for x in orc.gen_lognormal_sizes(21, 3000)] + [0, 1, 40_000]
+    n = int(sum(lengths))
+    off = _offsets(lengths)
+    keys = _u8(random_keys(0, 4, n, seed=21))
+    vals = _u8(index_values(4, n))
+    ek, ev = orc.segmented_sort(keys, vals, off, 0, 4, False, vb=4)
+    d = rs.make_desc(0, 4, 4, False)
+    doff = torch.from_numpy(off.astype(np.int64)).cuda()
+    tb = rs.segmented_sort_raw(d, None, None, n, None, 8, len(lengths), None, 0)
+    ws = torch.empty(tb, dtype=torch.uint8, device="cuda")
+    src_k, src_v = torch.from_numpy(keys.copy()).cuda(), torch.from_numpy(vals.copy()).cuda()
+    dk, dv = src_k.clone(), src_v.clone()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    print(f"[case] caller capture segmented n={n} segments={len(lengths)}", flush=True)
+    with torch.cuda.graph(g):
+        rs.segmented_sort_raw(d, dk.data_ptr(), dv.data_ptr(), n, doff.data_ptr(), 8, len(lengths), ws.data_ptr(), tb,
+                              torch.cuda.current_stream())
+    for r in range(REPLAYS):
+        dk.copy_(src_k)
+        dv.copy_(src_v)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(dk.cpu().numpy(), ek), f"replay {r}: keys differ from the oracle"
+        assert np.array_equal(dv.cpu().numpy(), ev), f"replay {r}: values differ from the oracle"
+
+
+def test_segmented_from_two_host_threads(rs, orc):
+    """Two host threads sort segmented batches at once, each on its own stream
+    and workspace, through the shared side streams: every result equals the
+    oracle."""
+    import threading
+
+    cases = []
+    for t in range(2):
+        lengths = [int(x) for x in orc.gen_lognormal_sizes(30 + t, 1500)] + [25_000 + t]
+        n = int(sum(lengths))
+        off = _offsets(lengths)
+        keys = _u8(random_keys(0, 4, n, seed=30 + t))
+        vals = _u8(index_values(4, n))
+        ek, ev = orc.segmented_sort(keys, vals, off, 0, 4, False, vb=4)
+        d = rs.make_desc(0, 4, 4, False)
+        tb = rs.segmented_sort_raw(d, None, None, n, None, 8, len(lengths), None, 0)
+        cases.append(dict(n=n, segs=len(lengths), d=d, tb=tb, ek=ek, ev=ev,
+                          doff=torch.from_numpy(off.astype(np.int64)).cuda(),
+                          ws=torch.empty(tb, dtype=torch.uint8, device="cuda"),
+                          src_k=torch.from_numpy(keys.copy()).cuda(), src_v=torch.from_numpy(vals.copy()).cuda(),
+                          stream=torch.cuda.Stream()))
+    errors = []
+
+    def worker(c):
+        try:
+            for _ in range(6):
+                k, v = c["src_k"].clone(), c["src_v"].clone()
+                torch.cuda.synchronize()
+                rs.segmented_sort_raw(c["d"], k.data_ptr(), v.data_ptr(), c["n"], c["doff"].data_ptr(), 8, c["segs"],
+                                      c["ws"].data_ptr(), c["tb"], c["stream"])
+                c["stream"].synchronize()
+                if not (np.array_equal(k.cpu().numpy(), c["ek"]) and np.array_equal(v.cpu().numpy(), c["ev"])):
+                    errors.append(f"n={c['n']}: result differs from the oracle")
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    print("[case] segmented sorts from two host threads", flush=True)
+    threads = [threading.Thread(target=worker, args=(c,)) for c in cases]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
