@@ -77,6 +77,21 @@ def test_bin_edges_u32_pairs(rs, orc, desc, offset_dtype):
     check(rs, orc, keys, vals, off, 0, 4, 4, desc)
 
 
+@pytest.mark.parametrize("kind,kb,vb", [(0, 8, 0), (1, 8, 8), (2, 8, 4), (0, 16, 8), (2, 16, 0), (0, 4, 0),
+                                         (1, 2, 0), (0, 1, 4), (2, 4, 8)])
+def test_bin_edges_every_width(rs, orc, kind, kb, vb):
+    """Segments of cap - 1, cap and cap + 1 keys for every LDS bin of the
+    width (capacities from rsort_segmented_plan), then one long segment."""
+    d = rs.make_desc(kind, kb, vb)
+    caps = list(rs.segmented_plan(d, 1 << 20, 8, 1).bin_capacity)
+    lengths = [0, 1, 2] + [c + dl for c in caps for dl in (-1, 0, 1)] + [3 * caps[-1] + 7]
+    n = int(sum(lengths)) + 3
+    keys = random_keys(kind, kb, n, seed=kb * 7 + vb + kind)
+    vals = random_values(vb, n, seed=vb + 1) if vb else None
+    off = offsets_from_lengths(lengths, start=3)
+    check(rs, orc, keys, vals, off, kind, kb, vb, False)
+
+
 @pytest.mark.parametrize("kind,kb", KEY_TYPES)
 @pytest.mark.parametrize("vb", [0, 4])
 def test_key_types(rs, orc, kind, kb, vb):
