@@ -617,6 +617,38 @@ rsort_status enqueue_sort(const Plan& p, void* keys, void* values, const SortBuf
     return RSORT_SUCCESS;
 }
 
+// Plain sorts of at most the largest LDS bin's capacity (16384 keys for
+// 4-byte keys and values) run as one workgroup of the segmented sort's LDS
+// bins: one launch instead of the histogram and a launch per pass (the same
+// stable LSD order, every pass in LDS).  RSORT_AMD_SMALL_LDS=0 keeps them on
+// the onesweep path; so do the profiled entry (its stage events) and values
+// moved by index.
+bool small_lds_allowed() {
+    static const bool v = [] {
+        const char* e = getenv("RSORT_AMD_SMALL_LDS");
+        return !(e && strcmp(e, "0") == 0);
+    }();
+    return v;
+}
+
+int small_lds_bin(const Plan& p, uint64_t n) {
+    if (p.index_mode || n < 2 || !small_lds_allowed()) return -1;
+    for (int b = 0; b < kSegBins; ++b)
+        if (n <= seg_cap((int)p.kb, (int)p.vb_core, b)) return b;
+    return -1;
+}
+
+hipError_t dispatch_req(const ReqJob& j, hipStream_t s) {
+    switch (j.kb) {
+        case 1: return launch_req_kb1(j, s);
+        case 2: return launch_req_kb2(j, s);
+        case 4: return launch_req_kb4(j, s);
+        case 8: return launch_req_kb8(j, s);
+        case 16: return launch_req_kb16(j, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* workspace, size_t* temp_bytes, uint64_t n,
                       void* stream, hipEvent_t* events, int max_events, int* n_events) {
     Plan p;
@@ -633,6 +665,28 @@ rsort_status run_sort(const rsort_desc* d, void* keys, void* values, void* works
     st = check_sort_args(p, keys, values);
     if (st != RSORT_SUCCESS) return st;
     if ((uintptr_t)workspace % 16) return fail(RSORT_ERROR_INVALID_ARGUMENT, "workspace not 16-byte aligned");
+
+    const int sbin = events ? -1 : small_lds_bin(p, n);
+    if (sbin >= 0) {
+        ReqJob j{};
+        j.kb = p.kb;
+        j.vb = p.vb_core;
+        j.tw = make_tw_masks(p.kind, p.kb, p.desc);
+        j.reqs = nullptr;
+        j.count[sbin] = 1u;
+        j.keys = keys;
+        j.vals = values;
+        j.n = n;
+        auto enqueue_small = [&](hipStream_t s) -> rsort_status {
+            const hipError_t e = dispatch_req(j, s);
+            return e == hipSuccess ? RSORT_SUCCESS : hip_fail(e, "small sort launch");
+        };
+        if (d->flags & RSORT_FLAG_GRAPH) {
+            const GraphKey k = graph_key(0u, d, keys, values, workspace, *temp_bytes, n, nullptr, 0u, 0u);
+            return run_graphed(k, stream, enqueue_small);
+        }
+        return enqueue_small((hipStream_t)stream);
+    }
 
     uint8_t* base = (uint8_t*)(((uintptr_t)workspace + 255u) & ~(uintptr_t)255u);
     SortBuffers b{};

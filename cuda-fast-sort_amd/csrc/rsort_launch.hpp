@@ -91,14 +91,18 @@ struct SegJob {
 };
 
 // Requests sorted in place by the LDS bins, one workgroup each (the serving
-// loop's direct form): reqs = SegReq list grouped by bin, count[b] of bin b
-// starting at first[b].  Every request fits its bin (n <= bin capacity).
+// loop's direct form, and plain sorts small enough for one bin): reqs =
+// SegReq list grouped by bin, count[b] of bin b starting at first[b].  Every
+// request fits its bin (n <= bin capacity).
 struct ReqJob {
     uint32_t kb, vb;
     TwMasks tw;
-    const void* reqs;
+    const void* reqs;     // null: one request, keys/vals/n below (count 1 in its bin)
     uint32_t first[kSegBins];
     uint32_t count[kSegBins];
+    void* keys;
+    void* vals;
+    uint64_t n;
 };
 
 // One launcher per key width, each in its own translation unit.

@@ -33,6 +33,9 @@ template <int KB, int VB>
 hipError_t launch_req_v(const ReqJob& j, hipStream_t s) {
     SegParams a{};
     a.reqs = (const SegReq*)j.reqs;
+    a.keys = j.keys;
+    a.vals = j.vals;
+    a.n = j.n;
     a.tw = j.tw;
     static_assert(kSegBins == 9, "one launch per bin below");
     launch_req_bin<KB, VB, 0>(a, j.first, j.count, s);

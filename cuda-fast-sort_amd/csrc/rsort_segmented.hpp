@@ -202,6 +202,7 @@ static __global__ __launch_bounds__(kSegChunk) void k_seg_fill(SegParams a) {
 // LDS bins: one workgroup sorts one segment of at most THREADS*K keys.
 // ---------------------------------------------------------------------------
 // DIRECT: workgroup b sorts request reqs[bin + b] (grid = the bin's request
+// count), or, without a request list, the one array keys/vals of n keys
 // count); otherwise a persistent workgroup takes segments of bin `bin` by ticket.
 template <int KB, int VB, int THREADS, int K, bool DIRECT>
 __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) {
@@ -270,11 +271,18 @@ __global__ __launch_bounds__(THREADS) void k_seg_lds(SegParams a, uint32_t bin) 
         uint32_t m;
         if constexpr (DIRECT) {
             if (round) break;
-            const SegReq r = a.reqs[bin + blockIdx.x];
-            keys = (KM*)r.keys;
-            vals = (VM*)r.vals;
+            if (a.reqs) {
+                const SegReq r = a.reqs[bin + blockIdx.x];
+                keys = (KM*)r.keys;
+                vals = (VM*)r.vals;
+                m = (uint32_t)r.n;
+            } else {
+                // no request list: one array of a.n keys (a plain sort that fits the bin)
+                keys = (KM*)a.keys;
+                vals = (VM*)a.vals;
+                m = (uint32_t)a.n;
+            }
             sb = 0;
-            m = (uint32_t)r.n;
         } else {
             // uniform: the segment's range and base pointers live in scalar registers
             const uint32_t item = __builtin_amdgcn_readfirstlane(s_item[round & 1u]);

@@ -1,8 +1,9 @@
 """Parity against the oracle of the code paths that environment variables
 select: the early tile aggregate on every u32/u32 pass (RSORT_AMD_EARLY_AGG_TILES=1),
 the ordinary lookback walk at every size (RSORT_AMD_LB_WIDE_TILES=0), the
-two-level histogram fold at every grid size (RSORT_AMD_HIST_DIRECT=0) and the
-single global tile ticket (RSORT_AMD_XCD_TICKETS=0).
+two-level histogram fold at every grid size (RSORT_AMD_HIST_DIRECT=0), the
+single global tile ticket (RSORT_AMD_XCD_TICKETS=0) and the onesweep path
+for sorts small enough for one LDS bin (RSORT_AMD_SMALL_LDS=0).
 
 The library reads them once per process, so test_gpu_env_paths.py runs this
 file in a process of its own per setting.  Cases: u32 keys and u32/u32 pairs
@@ -62,6 +63,11 @@ def main():
     r = orc.mt19937(3, 200_000)
     k_odd = ((r & 0xFF000000) | 0x00123456).astype(np.uint32)
     check(u8(k_odd), u8(index_values(4, 200_000)), 0, 4, 4, False)
+    # sorts small enough for one LDS bin (on the onesweep path with
+    # RSORT_AMD_SMALL_LDS=0): one histogram workgroup, one or few tiles
+    for kind, kb, vb, n in ((0, 2, 0, 9_000), (2, 4, 4, 16_384), (1, 8, 8, 5_003), (0, 16, 8, 4_096), (0, 1, 4, 777)):
+        check(u8(random_keys(kind, kb, n, seed=n + kb)), u8(index_values(vb, n)) if vb in (4, 8) else None,
+              kind, kb, vb if vb in (4, 8) else 0, False)
     print("env path parity ok", flush=True)
 
 

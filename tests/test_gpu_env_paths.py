@@ -16,6 +16,7 @@ SETTINGS = {
                                                      "RSORT_AMD_LB_WIDE_TILES": "0",
                                                      "RSORT_AMD_HIST_DIRECT": "0"},
     "global_tile_ticket": {"RSORT_AMD_XCD_TICKETS": "0"},
+    "small_sorts_on_onesweep": {"RSORT_AMD_SMALL_LDS": "0"},
 }
 
 
