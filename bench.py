@@ -659,9 +659,11 @@ def bench_serve(rs, a):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ids2 = srv.submit_many(kptrs, vptrs, [int(x) for x in sizes])
-    for i in ids2:
-        srv.wait(i)
+    infos2 = [srv.wait(i) for i in ids2]
     t_sat = time.perf_counter() - t0
+    # all submitted at once: the server finished the stream when its last
+    # request completed (latency = submit -> completion, timed by the server)
+    t_srv = max(x.latency_us for x in infos2) * 1e-6
     st2 = srv.stats()
     srv.close()
     return {
@@ -674,7 +676,10 @@ def bench_serve(rs, a):
         "offered_gkeys_per_s": round(n / a.duration / 1e9, 5),
         "achieved_gkeys_per_s": round(n / (t_end - t_start) / 1e9, 5),
         "batches": int(st.batches - st0.batches),
-        "saturation_gkeys_per_s": round(n / t_sat / 1e9, 3), "saturation_batches": int(st2.batches - st.batches),
+        "saturation_gkeys_per_s": round(n / t_srv / 1e9, 3),
+        "saturation_note": "every request submitted at once (submit_many); keys / the server's submit-to-last-"
+                           "completion time; saturation_wall_gkeys_per_s adds the host's 5000 wait calls",
+        "saturation_wall_gkeys_per_s": round(n / t_sat / 1e9, 3), "saturation_batches": int(st2.batches - st.batches),
         "checked": ok,
     }
 
