@@ -57,12 +57,24 @@ struct ServeHeader {
     uint64_t pad;
 };
 
-constexpr uint64_t kServeChunk = 65536;
+// Pieces of 4096 keys: a batch of a few long requests still spreads over
+// many workgroups (one piece per workgroup at a time).
+constexpr uint64_t kServeChunk = 4096;
 
-// Copy `count` elements of type E (a key or value word, or a slice of one).
+// Copy `count` elements of type E (a key or value word, or a slice of one):
+// four loads in flight per thread before their stores.
 template <typename E>
 __device__ __forceinline__ void serve_copy(E* __restrict__ dst, const E* __restrict__ src, uint64_t count) {
-    for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) dst[i] = src[i];
+    const uint64_t step = blockDim.x;
+    uint64_t i = threadIdx.x;
+    for (; i + 3 * step < count; i += 4 * step) {
+        const E a = src[i], b = src[i + step], c = src[i + 2 * step], d = src[i + 3 * step];
+        dst[i] = a;
+        dst[i + step] = b;
+        dst[i + 2 * step] = c;
+        dst[i + 3 * step] = d;
+    }
+    for (; i < count; i += step) dst[i] = src[i];
 }
 
 // Batch copy in: pieces -> batch buffer, and the segment offsets
@@ -187,6 +199,8 @@ struct Slot {
     uint8_t* reqs_dev = nullptr;
     void* flat_ws = nullptr;      // workspace of rsort_sort for long requests (direct form)
     size_t flat_ws_bytes = 0;
+    hipStream_t side[3] = {};     // direct form: the batch forks onto these and joins back
+    hipEvent_t fork = nullptr, join[3] = {};
     std::vector<uint64_t> ids;    // requests of the batch in flight
     std::vector<SegReq> items;    // their buffers and sizes
     uint64_t batch = 0;
@@ -220,7 +234,33 @@ struct Server {
     size_t chunks_bytes() const { return (size_t)max_chunks * sizeof(ServeChunk); }
     size_t host_bytes() const { return header_bytes() + chunks_bytes() + ((size_t)cfg.batch_requests + 1) * 8u; }
 
-    hipError_t enqueue_batch(Slot& sl) {
+    // Copy list of requests sl.items[pick[i]] (sl.items[i] without `pick`),
+    // i < count, into the slot's host-mapped header / pieces / offsets; the
+    // slot is idle, so nothing reads them meanwhile.  Returns the keys listed.
+    uint64_t fill_copy_list(Slot& sl, const size_t* pick, size_t count) {
+        ServeHeader* h = (ServeHeader*)sl.host;
+        ServeChunk* ch = (ServeChunk*)(sl.host + header_bytes());
+        uint64_t* off = (uint64_t*)(sl.host + header_bytes() + chunks_bytes());
+        uint64_t nch = 0, pos = 0;
+        for (size_t i = 0; i < count; ++i) {
+            const SegReq& r = sl.items[pick ? pick[i] : i];
+            off[i] = pos;
+            for (uint64_t b0 = 0; b0 < r.n; b0 += kServeChunk) {
+                ServeChunk& c = ch[nch++];
+                c.keys = r.keys + b0 * kb;
+                c.vals = vb ? r.vals + b0 * vb : 0;
+                c.len = r.n - b0 < kServeChunk ? r.n - b0 : kServeChunk;
+                c.offset = pos + b0;
+            }
+            pos += r.n;
+        }
+        h->chunks = nch;
+        h->requests = count;
+        h->total = pos;
+        return pos;
+    }
+
+    CopyArgs copy_args(Slot& sl) const {
         CopyArgs c{};
         c.hdr = (const ServeHeader*)sl.host_dev;
         c.chunks = (const ServeChunk*)(sl.host_dev + header_bytes());
@@ -231,6 +271,11 @@ struct Server {
         c.offsets = sl.offsets;
         c.cap_requests = cfg.batch_requests;
         c.grid = 1024;
+        return c;
+    }
+
+    hipError_t enqueue_batch(Slot& sl) {
+        const CopyArgs c = copy_args(sl);
         hipError_t e = launch_serve_copy(c, kb, ve, true, sl.stream);
         if (e != hipSuccess) return e;
         size_t tb = sl.ws_bytes;
@@ -240,8 +285,33 @@ struct Server {
         return launch_serve_copy(c, kb, ve, false, sl.stream);
     }
 
+    static constexpr size_t kForkRequests = 32;
+#ifndef RSA_SERVE_LONG_EACH
+#define RSA_SERVE_LONG_EACH 4
+#endif
+    // up to this many long requests in a batch are sorted one by one
+    static constexpr size_t kLongEach = RSA_SERVE_LONG_EACH;
+
+    hipError_t launch_req_on(const ReqJob& j, hipStream_t st) {
+        switch (kb) {
+            case 1: return launch_req_kb1(j, st);
+            case 2: return launch_req_kb2(j, st);
+            case 4: return launch_req_kb4(j, st);
+            case 8: return launch_req_kb8(j, st);
+            default: return launch_req_kb16(j, st);
+        }
+    }
+
     // Direct form: bin the slot's requests on the host, one launch per
-    // non-empty bin, rsort_sort for each request longer than the LDS capacity.
+    // non-empty bin.  A request longer than the LDS capacity is sorted in
+    // place by rsort_sort; two or more are copied into the slot's batch
+    // buffer back to back, sorted there by one rsort_segmented_sort (its long
+    // segment path: a fixed number of launches for any number of requests)
+    // and copied back.  The requests are disjoint, so a large batch forks onto
+    // the slot's three side streams: the bins spread over the four streams as
+    // the segmented sort spreads them, the long requests go on the one with
+    // the least bin work, and the side streams join back before the slot's
+    // completion event is recorded.
     hipError_t enqueue_direct(Slot& sl) {
         ReqJob j{};
         j.kb = kb;
@@ -251,6 +321,7 @@ struct Server {
         const size_t nr = sl.items.size();
         std::vector<uint32_t> bin(nr);
         uint32_t in_bin[kSegBins] = {};
+        std::vector<size_t> longs;
         for (size_t i = 0; i < nr; ++i) {
             const uint64_t n = sl.items[i].n;
             uint32_t b = kSegBins;  // longer than every bin (or nothing to do)
@@ -262,6 +333,7 @@ struct Server {
                     }
             bin[i] = b;
             if (b < (uint32_t)kSegBins) ++in_bin[b];
+            else if (n >= 2) longs.push_back(i);
         }
         uint32_t at = 0;
         for (int b = 0; b < kSegBins; ++b) {
@@ -272,21 +344,90 @@ struct Server {
         SegReq* list = (SegReq*)sl.reqs;  // host-mapped; the slot is idle
         for (size_t i = 0; i < nr; ++i)
             if (bin[i] < (uint32_t)kSegBins) list[j.first[bin[i]] + j.count[bin[i]]++] = sl.items[i];
+
+        // fork only for a batch large enough to pay for the cross-stream
+        // events, and only when at least two streams get work: batches of a
+        // few requests (light load) stay on one stream.  Stream 1 takes the
+        // long requests (its one bin, 5, is short).
+        static constexpr int kBinStream[kSegBins] = {3, 2, 3, 2, 2, 1, 0, 0, 3};  // bins 7, 6 | 5 | 4, 3, 1 | 0, 2, 8
+        static constexpr int kLongStream = 1;
+        const bool long_batch = longs.size() > kLongEach;
+        bool busy[4] = {false, false, false, false};
+        for (int b = 0; b < kSegBins; ++b) busy[kBinStream[b]] = busy[kBinStream[b]] || j.count[b] != 0;
+        if (long_batch) busy[kLongStream] = true;
+        for (size_t q = 0; q < longs.size() && !long_batch; ++q) busy[(kLongStream + q) % 4] = true;
+        const int nbusy = (int)busy[0] + (int)busy[1] + (int)busy[2] + (int)busy[3];
+        const bool fork = sl.side[0] != nullptr && nbusy > 1 && (nr >= kForkRequests || longs.size() > 1);
+        const hipStream_t st[4] = {sl.stream, fork ? sl.side[0] : sl.stream, fork ? sl.side[1] : sl.stream,
+                                   fork ? sl.side[2] : sl.stream};
         hipError_t e = hipSuccess;
-        switch (kb) {
-            case 1: e = launch_req_kb1(j, sl.stream); break;
-            case 2: e = launch_req_kb2(j, sl.stream); break;
-            case 4: e = launch_req_kb4(j, sl.stream); break;
-            case 8: e = launch_req_kb8(j, sl.stream); break;
-            default: e = launch_req_kb16(j, sl.stream); break;
+        if (fork) {
+            e = hipEventRecord(sl.fork, sl.stream);
+            for (int k = 0; k < 3 && e == hipSuccess; ++k)
+                if (busy[k + 1]) e = hipStreamWaitEvent(sl.side[k], sl.fork, 0);
         }
-        for (size_t i = 0; i < nr && e == hipSuccess; ++i) {
-            const SegReq& r = sl.items[i];
-            if (bin[i] < (uint32_t)kSegBins || r.n < 2) continue;
-            size_t tb = sl.flat_ws_bytes;
-            if (rsort_sort(&cfg.desc, (void*)(uintptr_t)r.keys, (void*)(uintptr_t)r.vals, sl.flat_ws, &tb, r.n,
-                           sl.stream) != RSORT_SUCCESS)
+        // long requests first: theirs is the longest chain of launches.  A
+        // few: one rsort_sort each, round-robin over the streams from
+        // kLongStream on, each stream's in its own part of the flat workspace
+        // (all on one stream when the parts do not fit).  More: one
+        // segmented sort in the batch buffer.
+        if (e == hipSuccess && !long_batch && !longs.empty()) {
+            int nst = fork ? 4 : 1;
+            uint64_t maxn[4] = {0, 0, 0, 0};
+            size_t off[4] = {0, 0, 0, 0}, bytes[4] = {0, 0, 0, 0}, total = 0;
+            for (size_t q = 0; q < longs.size(); ++q) {
+                const uint64_t n = sl.items[longs[q]].n;
+                if (n > maxn[q % nst]) maxn[q % nst] = n;
+            }
+            for (int k = 0; k < nst; ++k) {
+                if (!maxn[k]) continue;
+                size_t need = 0;
+                if (rsort_sort(&cfg.desc, nullptr, nullptr, nullptr, &need, maxn[k], nullptr) != RSORT_SUCCESS)
+                    need = sl.flat_ws_bytes + 1u;
+                off[k] = total;
+                bytes[k] = need;
+                total += (need + 255u) & ~(size_t)255u;
+            }
+            if (total > sl.flat_ws_bytes) {
+                nst = 1;
+                off[0] = 0;
+                bytes[0] = sl.flat_ws_bytes;
+            }
+            for (size_t q = 0; q < longs.size() && e == hipSuccess; ++q) {
+                const int k = (int)(q % nst);
+                const SegReq& r = sl.items[longs[q]];
+                size_t tb = bytes[k];
+                if (rsort_sort(&cfg.desc, (void*)(uintptr_t)r.keys, (void*)(uintptr_t)r.vals,
+                               (uint8_t*)sl.flat_ws + off[k], &tb, r.n, st[(kLongStream + k) % 4]) != RSORT_SUCCESS)
+                    e = hipErrorLaunchFailure;
+            }
+        } else if (e == hipSuccess && long_batch) {
+            const uint64_t total = fill_copy_list(sl, longs.data(), longs.size());
+            const CopyArgs c = copy_args(sl);
+            e = launch_serve_copy(c, kb, ve, true, st[kLongStream]);
+            rsort_desc d = cfg.desc;
+            d.flags &= ~RSORT_FLAG_GRAPH;  // sizes differ from batch to batch
+            size_t tb = sl.ws_bytes;
+            if (e == hipSuccess && rsort_segmented_sort(&d, sl.bkeys, sl.bvals, total, sl.offsets, 8, longs.size(),
+                                                        sl.ws, &tb, st[kLongStream]) != RSORT_SUCCESS)
                 e = hipErrorLaunchFailure;
+            if (e == hipSuccess) e = launch_serve_copy(c, kb, ve, false, st[kLongStream]);
+        }
+        for (int k = 0; k < 4 && e == hipSuccess; ++k) {
+            ReqJob jk = j;
+            bool any = false;
+            for (int b = 0; b < kSegBins; ++b) {
+                if (kBinStream[b] != k) jk.count[b] = 0;
+                any = any || jk.count[b] != 0;
+            }
+            if (any) e = launch_req_on(jk, st[k]);
+        }
+        if (fork) {
+            for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+                if (!busy[k + 1]) continue;
+                e = hipEventRecord(sl.join[k], sl.side[k]);
+                if (e == hipSuccess) e = hipStreamWaitEvent(sl.stream, sl.join[k], 0);
+            }
         }
         return e;
     }
@@ -315,6 +456,21 @@ struct Server {
         }
         hipError_t e = enqueue_direct(sl);
         uint64_t n = longest + 1u > 2u ? longest + 1u : 2u;
+        // enough long requests for one segmented sort: the copy and
+        // segmented-sort kernels (each copy lands where it came from, the
+        // requests lying back to back in the batch buffer)
+        const uint64_t nlong = kLongEach + 1u;
+        if (e == hipSuccess && nlong * n <= cfg.batch_keys) {
+            sl.items.clear();
+            for (uint64_t i = 0; i < nlong; ++i) {
+                SegReq r{};
+                r.keys = (uint64_t)(uintptr_t)((uint8_t*)sl.bkeys + i * n * kb);
+                r.vals = vb ? (uint64_t)(uintptr_t)((uint8_t*)sl.bvals + i * n * vb) : 0u;
+                r.n = n;
+                sl.items.push_back(r);
+            }
+            e = enqueue_direct(sl);
+        }
         while (n <= cfg.batch_keys && e == hipSuccess) {
             size_t tb = sl.flat_ws_bytes;
             if (rsort_sort(&cfg.desc, sl.bkeys, sl.bvals, sl.flat_ws, &tb, n, sl.stream) != RSORT_SUCCESS)
@@ -363,53 +519,31 @@ struct Server {
             free_slots.pop_back();
             Slot& sl = slots[si];
             // admission into this batch: FIFO while the slot has room
+            // (with a snapshot of each record: they are only read under the lock)
             sl.ids.clear();
+            sl.items.clear();
+            sl.batch = ++batches;
+            const auto now = Clock::now();
             uint64_t keys = 0;
             while (!pending.empty() && sl.ids.size() < cfg.batch_requests) {
-                Request& r = req[pending.front()];
+                Request& r = req.find(pending.front())->second;
                 if (keys + r.n > cfg.batch_keys) break;
                 keys += r.n;
                 pending_keys -= r.n;
+                r.t_launch = now;
+                r.batch = sl.batch;
                 sl.ids.push_back(pending.front());
+                sl.items.push_back(SegReq{(uint64_t)(uintptr_t)r.keys, (uint64_t)(uintptr_t)r.vals, r.n, 0});
                 pending.pop_front();
             }
             if (pending.empty()) flush = false;
             ++launching;
-            sl.batch = ++batches;
-            const auto now = Clock::now();
-            // snapshot of the batch (the records are only read under the lock)
-            sl.items.resize(sl.ids.size());
-            for (size_t i = 0; i < sl.ids.size(); ++i) {
-                Request& r = req[sl.ids[i]];
-                r.t_launch = now;
-                r.batch = sl.batch;
-                sl.items[i] = SegReq{(uint64_t)(uintptr_t)r.keys, (uint64_t)(uintptr_t)r.vals, r.n, 0};
-            }
             lk.unlock();
             hipError_t e = hipSuccess;
             if (direct) {
                 e = enqueue_direct(sl);
             } else {
-                // copy list (host-mapped; the slot is idle, so nothing reads it)
-                ServeHeader* h = (ServeHeader*)sl.host;
-                ServeChunk* ch = (ServeChunk*)(sl.host + header_bytes());
-                uint64_t* off = (uint64_t*)(sl.host + header_bytes() + chunks_bytes());
-                uint64_t nch = 0, pos = 0;
-                for (size_t i = 0; i < sl.items.size(); ++i) {
-                    const SegReq& r = sl.items[i];
-                    off[i] = pos;
-                    for (uint64_t b0 = 0; b0 < r.n; b0 += kServeChunk) {
-                        ServeChunk& c = ch[nch++];
-                        c.keys = r.keys + b0 * kb;
-                        c.vals = vb ? r.vals + b0 * vb : 0;
-                        c.len = r.n - b0 < kServeChunk ? r.n - b0 : kServeChunk;
-                        c.offset = pos + b0;
-                    }
-                    pos += r.n;
-                }
-                h->chunks = nch;
-                h->requests = sl.items.size();
-                h->total = pos;
+                fill_copy_list(sl, nullptr, sl.items.size());
                 e = sl.exec ? hipGraphLaunch(sl.exec, sl.stream) : enqueue_batch(sl);
             }
             if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
@@ -475,6 +609,11 @@ void destroy_slots(Server* s) {
     for (auto& sl : s->slots) {
         if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
         if (sl.done) (void)hipEventDestroy(sl.done);
+        for (int k = 0; k < 3; ++k) {
+            if (sl.side[k]) (void)hipStreamDestroy(sl.side[k]);
+            if (sl.join[k]) (void)hipEventDestroy(sl.join[k]);
+        }
+        if (sl.fork) (void)hipEventDestroy(sl.fork);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         if (sl.bkeys) (void)hipFree(sl.bkeys);
         if (sl.bvals) (void)hipFree(sl.bvals);
@@ -519,7 +658,12 @@ rsort_status rsort_serve_create(const rsort_serve_config* cfg, rsort_server** ou
     }
     size_t flat_need = 0;
     if (s->direct) {
+        // room for the long requests of a batch split over four streams: their
+        // keys add up to at most a batch, each part adds its fixed overhead
+        size_t fixed = 0;
         st = rsort_sort(&cfg->desc, nullptr, nullptr, nullptr, &flat_need, cfg->batch_keys, nullptr);
+        if (st == RSORT_SUCCESS) st = rsort_sort(&cfg->desc, nullptr, nullptr, nullptr, &fixed, 1, nullptr);
+        flat_need += 3 * (fixed + 256u);
         if (st != RSORT_SUCCESS) {
             delete s;
             return st;
@@ -530,6 +674,7 @@ rsort_status rsort_serve_create(const rsort_serve_config* cfg, rsort_server** ou
         return RSORT_ERROR_DEVICE;
     }
     s->slots.resize(cfg->depth);
+    s->req.reserve((size_t)cfg->batch_requests * (cfg->depth + 1u));  // no rehash while batches fill
     hipError_t e = hipSuccess;
     for (uint32_t i = 0; i < cfg->depth && e == hipSuccess; ++i) {
         auto& sl = s->slots[i];
@@ -550,6 +695,11 @@ rsort_status rsort_serve_create(const rsort_serve_config* cfg, rsort_server** ou
             if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&sl.reqs_dev, sl.reqs, 0);
             sl.flat_ws_bytes = flat_need;
             if (e == hipSuccess) e = hipMalloc(&sl.flat_ws, flat_need);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.fork, hipEventDisableTiming);
+            for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+                e = hipStreamCreateWithFlags(&sl.side[k], hipStreamNonBlocking);
+                if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.join[k], hipEventDisableTiming);
+            }
         }
     }
     if (e == hipSuccess && s->direct) e = s->warm_direct(s->slots[0]);

@@ -426,11 +426,17 @@ class Server:
         h = ctypes.c_void_p()
         _check(lib().rsort_serve_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
+        self.key_bytes = key_type.nbytes
         self.value_bytes = value_bytes
 
     def submit(self, keys, values=None, n: int | None = None) -> int:
         rid = ctypes.c_uint64(0)
         n = int(keys.shape[0]) if n is None else int(n)
+        # tensors are checked for room; any other object with data_ptr() is a
+        # raw device address the caller vouches for
+        for t, name, b in ((keys, "keys", self.key_bytes), (values, "values", self.value_bytes)):
+            if hasattr(t, "numel"):
+                _check_tensor(t, name, n * b)
         _check(lib().rsort_serve_submit(self._h, _ptr(keys), _ptr(values), n, ctypes.byref(rid)))
         return int(rid.value)
 

@@ -131,3 +131,41 @@ def test_submit_many(rs, orc):
     for k, d_k, d_v in zip(host, dk, dv):
         ek, ev = orc.sort(k, np.arange(k.size, dtype=np.int32), 0, 4, False, vb=4)
         assert np.array_equal(d_k.cpu().numpy(), ek) and np.array_equal(d_v.cpu().numpy(), ev)
+
+
+@pytest.mark.parametrize("kind,kb,vb,desc", [(0, 4, 4, False), (1, 4, 0, True), (0, 8, 8, False), (2, 8, 4, True)])
+def test_many_long_requests_in_one_batch(rs, orc, kind, kb, vb, desc):
+    """One direct-form batch with a dozen requests longer than every LDS bin
+    (copied into the batch buffer and sorted by one segmented sort), short
+    requests in every bin beside them, and one batch with a single long
+    request (sorted in place); each request equals the oracle's sort."""
+    from helpers import random_keys, random_values
+
+    print(f"[case] serve long batch kind={kind} kb={kb} vb={vb} desc={desc}", flush=True)
+    rng = np.random.default_rng(40 + kb + vb)
+    sizes = [int(x) for x in rng.integers(16385, 120000, size=12)]
+    sizes += [int(x) for x in rng.integers(0, 16384, size=60)]
+    rng.shuffle(sizes)
+    host_k = [random_keys(kind, kb, m, seed=100 + i) for i, m in enumerate(sizes)]
+    host_v = [random_values(vb, m, seed=200 + i) if vb else None for i, m in enumerate(sizes)]
+    dk = [torch.from_numpy(k.copy()).cuda() for k in host_k]
+    dv = [torch.from_numpy(v.copy()).cuda() if vb else None for v in host_v]
+    lone_k = random_keys(kind, kb, 50000, seed=7)
+    lone_v = random_values(vb, 50000, seed=8) if vb else None
+    dlk = torch.from_numpy(lone_k.copy()).cuda()
+    dlv = torch.from_numpy(lone_v.copy()).cuda() if vb else None
+    torch.cuda.synchronize()
+    with rs.Server(rs.KeyType(kind, kb), value_bytes=vb, descending=desc, batch_keys=1 << 22, batch_requests=128,
+                   depth=2, mode="direct") as srv:
+        ids = srv.submit_many([k.data_ptr() for k in dk], [v.data_ptr() for v in dv] if vb else None, sizes)
+        infos = [srv.wait(i) for i in ids]
+        lone = srv.wait(srv.submit(dlk, dlv, n=50000))  # the helpers' arrays are bytes
+        assert srv.stats().direct == 1
+    assert len({x.batch for x in infos}) == 1  # one batch held them all
+    assert lone.batch != infos[0].batch
+    for i, (k, v, m) in enumerate(zip(host_k + [lone_k], host_v + [lone_v], sizes + [50000])):
+        ek, ev = orc.sort(k, v, kind, kb, desc, vb=vb if vb else None)
+        got_k = (dk + [dlk])[i].cpu().numpy()
+        assert np.array_equal(got_k, ek), f"request {i} (n={m}) keys"
+        if vb:
+            assert np.array_equal((dv + [dlv])[i].cpu().numpy(), ev), f"request {i} (n={m}) values"
