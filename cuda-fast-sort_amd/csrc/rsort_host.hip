@@ -15,7 +15,12 @@
 
 #include <dlfcn.h>
 
+#include <unistd.h>
+
+#include <atomic>
+#include <chrono>
 #include <mutex>
+#include <random>
 #include <vector>
 
 #include "../../include/rsort_c.h"
@@ -79,6 +84,34 @@ bool xcd_tickets_allowed() {
         return !(e && strcmp(e, "0") == 0);
     }();
     return v;
+}
+
+// The histogram kernel clears the control region itself (no fill kernel)
+// where its grid allows; RSORT_AMD_HIST_CLEAR=0 keeps the fill kernel.
+bool hist_clear_allowed() {
+    static const bool v = [] {
+        const char* e = getenv("RSORT_AMD_HIST_CLEAR");
+        return !(e && strcmp(e, "0") == 0);
+    }();
+    return v;
+}
+
+// A fresh nonzero 64-bit id per sort call: odd numbers counted up from a
+// random start (random device, clock and process id), so ids never repeat in
+// a process and one left in a workspace by another process matches by chance
+// only.
+uint64_t next_clear_id() {
+    static std::atomic<uint64_t> next{[] {
+        uint64_t v = (uint64_t)std::chrono::steady_clock::now().time_since_epoch().count() * 0x9E3779B97F4A7C15ull;
+        v ^= (uint64_t)getpid() << 32;
+        try {
+            std::random_device rd;
+            v ^= ((uint64_t)rd() << 32) ^ (uint64_t)rd();
+        } catch (...) {
+        }
+        return v | 1u;
+    }()};
+    return next.fetch_add(2u, std::memory_order_relaxed);
 }
 
 // Histogram grids of more than one fold group and at most this many
@@ -580,6 +613,14 @@ rsort_status enqueue_sort(const Plan& p, void* keys, void* values, const SortBuf
     j.early_agg = p.kb == 4 && p.vb_core == 4 && !p.wide && early_agg_min_tiles() > 0 &&
                   p.num_tiles >= early_agg_min_tiles();
     j.hist_direct_max = hist_direct_max();
+    // the histogram may clear the control region itself unless the launch is
+    // being captured (a graph's replays would all carry this one id)
+    j.clear_id = 0u;
+    if (hist_clear_allowed()) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusActive;
+        if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) j.clear_id = next_clear_id();
+        else (void)hipGetLastError();
+    }
     j.tw = make_tw_masks(p.kind, p.kb, p.desc);
     j.keys = keys;
     j.tmp_keys = b.tmp_keys;

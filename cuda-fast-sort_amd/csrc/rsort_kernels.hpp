@@ -378,8 +378,10 @@ struct HistParams {
     uint32_t passes;
     uint32_t allow_skip;
     uint32_t direct;       // fold by device atomics from every workgroup (small grids)
-    uint4* self_clear;     // grid of one workgroup: it clears this region (control header, counts,
-    uint64_t self_clear_units;  // tickets, lookback) itself, in 16-byte units; no fill kernel runs
+    uint4* self_clear;     // no fill kernel: the histogram clears this region (control header, counts,
+    uint64_t self_clear_units;  // tickets, lookback) itself, in 16-byte units
+    uint64_t head_units;   // its first part, header and counts (16-byte units)
+    uint64_t clear_id;     // several workgroups clearing: this call's id (nonzero), see k_histogram
     uint32_t* xcd_lock;    // the device's per-XCD ticket lock word (null: never taken)
     uint32_t want_xcd;     // the passes can run on per-XCD tickets (see k_onesweep)
     TwMasks tw;
@@ -618,7 +620,7 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
 
     for (uint32_t i = tid; i < HWORDS; i += kHistThreads) s_hist[i] = 0u;
     if (tid == 0) s_skip = 0u;
-    if (a.self_clear_units) {
+    if (a.self_clear_units && gridDim.x == 1u) {
         // the only workgroup: nothing else reads the control header, counts,
         // tickets or lookback words before the passes, so it clears them here.
         // The header goes through thread 0, which later stores the skip mask
@@ -627,6 +629,28 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
         if (tid == 0)
             for (uint32_t u = 0; u < hdr; ++u) a.self_clear[u] = make_uint4(0u, 0u, 0u, 0u);
         for (uint64_t u = hdr + tid; u < a.self_clear_units; u += kHistThreads)
+            a.self_clear[u] = make_uint4(0u, 0u, 0u, 0u);
+    } else if (a.self_clear_units) {
+        // Several workgroups and no fill kernel.  Workgroup 0 clears the
+        // header and the counts with coherent (write-through) stores, waits
+        // for them to complete, then stores this call's id into the header;
+        // every workgroup waits to read that id before its first access to
+        // the header or the counts (below), so the id a stale header may hold
+        // is never this call's (ids are unique per process).  Workgroup 0 is
+        // dispatched first and waits on nothing, so the others' wait ends.
+        // The ticket and lookback words, read only by the pass kernels, are
+        // cleared by every workgroup with plain stores.
+        if (blockIdx.x == 0) {
+            uint32_t* w32 = (uint32_t*)a.self_clear;
+            const uint32_t words = (uint32_t)(a.head_units * 4u);
+            for (uint32_t i = tid; i < words; i += kHistThreads)
+                __hip_atomic_store(w32 + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (tid == 0) __hip_atomic_store(&a.ctrl->clear_id, a.clear_id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        for (uint64_t u = a.head_units + (uint64_t)blockIdx.x * kHistThreads + tid; u < a.self_clear_units;
+             u += (uint64_t)gridDim.x * kHistThreads)
             a.self_clear[u] = make_uint4(0u, 0u, 0u, 0u);
     }
     __syncthreads();
@@ -719,6 +743,13 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     }
     __syncthreads();
     RSA_MARK(16);
+    if (a.clear_id) {
+        // the header and the counts are cleared for this call (see above)
+        if (tid == 0)
+            while (__hip_atomic_load(&a.ctrl->clear_id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.clear_id)
+                __builtin_amdgcn_s_sleep(1);
+        __syncthreads();
+    }
 
     Off* counts = (Off*)a.counts;
     const uint32_t groups = (gridDim.x + kHistGroup - 1) / kHistGroup;
@@ -1055,7 +1086,10 @@ __global__ __launch_bounds__(THREADS, pass_min_waves(KB, VB, K)) void k_onesweep
     constexpr int LB_WMAX = RSA_LB_WMAX;
     __shared__ OnesweepLds<KB, VB, WIDE, THREADS, K> L;
 
-    const PassRun pr = pass_run(a.ctrl->skip_mask, a.passes, a.pass);
+    // the two header words every pass reads first, in one load (the second,
+    // xcd_mode, picks the ticket counter below)
+    const uint2 hdr = *(const uint2*)&a.ctrl->skip_mask;
+    const PassRun pr = pass_run(hdr.x, a.passes, a.pass);
     if (!pr.runs) return;  // uniform digit: pass is the identity
     PassIo<KB, VB> io;
     io.kin = (const KM*)((pr.e & 1u) ? a.keys_b : a.keys_a);
@@ -1093,7 +1127,7 @@ __global__ __launch_bounds__(THREADS, pass_min_waves(KB, VB, K)) void k_onesweep
     // each fill XCDs with workgroups waiting on the other's undispatched
     // tiles.  The grid is whole rounds of parts * C tiles when xcd_ok:
     // workgroups past the last tile end here.
-    const bool xcd = a.xcd_ok && a.ctrl->xcd_mode;
+    const bool xcd = a.xcd_ok && hdr.y;
     const uint32_t parts = xcd ? kTicketParts : 1u;
     const uint32_t chunk = xcd ? tile_chunk(KB) : 1u;
     const uint32_t part = blockIdx.x % parts;

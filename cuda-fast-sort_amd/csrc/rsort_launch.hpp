@@ -27,6 +27,7 @@ struct SortJob {
     bool xcd_tickets;     // per-XCD tile tickets allowed (see pass_variant for when they are used)
     uint32_t* xcd_lock;   // the device's per-XCD ticket lock word (device address; null: global ticket only)
     uint32_t hist_direct_max;  // histogram grids up to this fold by direct device atomics
+    uint64_t clear_id;    // nonzero: the histogram may clear the control region itself (unique per call)
     TwMasks tw;
     void* keys;           // caller buffer (in/out)
     void* vals;           // caller buffer (in/out) or index buffer

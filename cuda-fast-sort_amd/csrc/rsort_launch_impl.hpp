@@ -73,13 +73,16 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     record_event(j, s);
     // 1. control header, global digit counts, tickets and lookback words
     //    (flag 0: never ready in any epoch), contiguous in the workspace: one
-    //    fill, or none when the histogram is a single workgroup (it clears
-    //    them itself)
-    const uint64_t clear_units =
-        (uint64_t)((sizeof(Ctrl) + (size_t)P * kRadix * sizeof(Off) + j.lb_bytes) / 16u);
+    //    fill, or none when the histogram clears them itself (a single
+    //    workgroup, or a grid within one workgroup per CU given a call id;
+    //    the fill stays for large grids and for captured launches, whose
+    //    replays would repeat one id)
+    const uint64_t head_units = (uint64_t)((sizeof(Ctrl) + (size_t)P * kRadix * sizeof(Off)) / 16u);
+    const uint64_t clear_units = head_units + (uint64_t)(j.lb_bytes / 16u);
     const uint32_t hg = grid_for(j.n, hist_keys_per_block_iter(KB), j.num_cus, j.hist_bpc ? j.hist_bpc : 1u);
     const uint32_t hgrid = hg < kHistMaxGrid ? hg : kHistMaxGrid;
-    if (hgrid > 1u)
+    const bool self_clear = hgrid == 1u || (j.clear_id != 0u && hgrid <= (uint32_t)j.num_cus);
+    if (!self_clear)
         hipLaunchKernelGGL(k_fill_zero16, dim3(grid_for(clear_units, 256u * 8u, j.num_cus, 4)), dim3(256), 0, s,
                            (uint4*)j.ctrl, clear_units);
     record_event(j, s);
@@ -101,8 +104,10 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     h.passes = P;
     h.allow_skip = j.allow_skip ? 1u : 0u;
     h.direct = (hgrid > (uint32_t)kHistGroup && hgrid <= j.hist_direct_max) ? 1u : 0u;
-    h.self_clear = hgrid == 1u ? (uint4*)j.ctrl : nullptr;
-    h.self_clear_units = hgrid == 1u ? clear_units : 0u;
+    h.self_clear = self_clear ? (uint4*)j.ctrl : nullptr;
+    h.self_clear_units = self_clear ? clear_units : 0u;
+    h.head_units = head_units;
+    h.clear_id = self_clear && hgrid > 1u ? j.clear_id : 0u;
     const PassKernel pk = choose_pass_kernel<KB, VB, WIDE>(j);
     h.xcd_lock = pk.xcd_ok ? j.xcd_lock : nullptr;
     h.want_xcd = pk.xcd_ok ? 1u : 0u;

@@ -30,13 +30,19 @@ constexpr uint32_t kMaxPasses = 16;
 // ---------------------------------------------------------------------------
 struct Ctrl {
     uint32_t ticket[kMaxPasses];  // per-pass tile tickets (dynamic tile order)
-    uint32_t hist_done;           // histogram workgroup groups finished
+    // skip_mask and xcd_mode side by side (8-byte aligned): every pass
+    // workgroup reads them first, in one scalar load
     uint32_t skip_mask;           // bit p set: every key has the same digit in pass p
-    uint32_t group_done[32];      // histogram workgroups finished per group
     uint32_t xcd_mode;            // 1: this sort holds the device's per-XCD ticket lock
-    uint32_t reserved[13];
+    uint32_t hist_done;           // histogram workgroup groups finished
+    uint32_t pad;
+    uint64_t clear_id;            // id of the call whose histogram cleared this header (no fill kernel)
+    uint32_t group_done[32];      // histogram workgroups finished per group
+    uint32_t reserved[10];
 };
 static_assert(sizeof(Ctrl) == 256, "control header must stay 256 bytes");
+static_assert(offsetof(Ctrl, skip_mask) % 8 == 0 && offsetof(Ctrl, xcd_mode) == offsetof(Ctrl, skip_mask) + 4,
+              "skip_mask and xcd_mode form one aligned 8-byte word");
 
 // Passes that run.  A pass whose digit is the same for every key is the
 // identity and is skipped, except that when the passes left are odd in

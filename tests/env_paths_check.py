@@ -2,8 +2,10 @@
 select: the early tile aggregate on every u32/u32 pass (RSORT_AMD_EARLY_AGG_TILES=1),
 the ordinary lookback walk at every size (RSORT_AMD_LB_WIDE_TILES=0), the
 two-level histogram fold at every grid size (RSORT_AMD_HIST_DIRECT=0), the
-single global tile ticket (RSORT_AMD_XCD_TICKETS=0) and the onesweep path
-for sorts small enough for one LDS bin (RSORT_AMD_SMALL_LDS=0).
+single global tile ticket (RSORT_AMD_XCD_TICKETS=0), the onesweep path
+for sorts small enough for one LDS bin (RSORT_AMD_SMALL_LDS=0) and the fill
+kernel clearing the control region before every histogram
+(RSORT_AMD_HIST_CLEAR=0).
 
 The library reads them once per process, so test_gpu_env_paths.py runs this
 file in a process of its own per setting.  Cases: u32 keys and u32/u32 pairs

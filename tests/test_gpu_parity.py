@@ -210,6 +210,33 @@ def test_workspace_reuse_and_stream(rs, orc):
             np.testing.assert_array_equal(dv.cpu().numpy(), ev)
 
 
+def test_workspace_garbage_and_back_to_back(rs, orc):
+    """Sorts whose histogram clears the control region itself (grids of 2 to
+    256 workgroups): a workspace first filled with all-ones bytes (every
+    lookback flag set, a header and counts of garbage), then three sorts
+    queued back to back on it with no synchronisation between them."""
+    ws = torch.full((64 << 20,), 0xFF, dtype=torch.uint8, device="cuda")
+    cases = [(4, 4, (1 << 20) + 7), (8, 0, 300_001), (4, 0, 1 << 21)]
+    dks, dvs, host = [], [], []
+    for i, (kb, vb, n) in enumerate(cases):
+        print(f"[case] garbage workspace kb={kb} vb={vb} n={n}", flush=True)
+        keys = random_keys(0, kb, n, seed=50 + i)
+        vals = random_values(vb, n, seed=60 + i) if vb else None
+        dk, dv = dev(keys), dev(vals)
+        d = rs.make_desc(0, kb, vb)
+        assert rs.sort_raw(d, None, None, None, 0, n) <= ws.numel()
+        rs.sort_raw(d, dk.data_ptr(), dv.data_ptr() if dv is not None else None, ws.data_ptr(), ws.numel(), n)
+        dks.append(dk)
+        dvs.append(dv)
+        host.append((keys, vals, kb, vb))
+    torch.cuda.synchronize()
+    for (keys, vals, kb, vb), dk, dv in zip(host, dks, dvs):
+        ek, ev = orc.sort(keys, vals, 0, kb, False, vb=vb if vb else None)
+        np.testing.assert_array_equal(dk.cpu().numpy(), ek)
+        if vb:
+            np.testing.assert_array_equal(dv.cpu().numpy(), ev)
+
+
 def test_python_api_mirror(rs, orc):
     """rsort_amd.sort / sort_pairs_descending with torch dtypes (example_basic flow)."""
     n = 50_000
