@@ -847,7 +847,7 @@ def main():
         out["plain20_us"] = out["configs"]["plain20"]["us_per_sort"]
         out["size_sweep"] = bench_size_sweep(rs)
         print("[bench] size sweep done", file=sys.stderr, flush=True)
-        rnd =bench_init_keys_mode(rs, 1, 1 << 27, max(3, a.steps // 2), 2)
+        rnd = bench_init_keys_mode(rs, 1, 1 << 27, max(3, a.steps // 2), 2)
         bsk = bench_init_keys_mode(rs, 2, 1 << 27, max(3, a.steps // 2), 2)
         out["byte_skip"] = {"description": "u32 n=2^27 from the reference generator: random vs byte_skip (every "
                                            "other byte constant), steady state", "random": rnd, "byte_skip": bsk}
