@@ -125,7 +125,9 @@ hipError_t launch_seg_t(const SegJob& j, hipStream_t s) {
     // 0.25-0.6 ms, the long segments ~0.5 ms).  Measured on MI355X, 100k
     // lognormal segments, u32/u32 (GKeys/s): nine bins with 3,0 | 5,8 | 4,2
     // | 7,6,1,long 100.4; 7 | 5,long | 4,3,1 | 0,2,8,6 102.3, 103.3 with the
-    // large bins exchanging keys and values separately; this one 105.0.
+    // large bins exchanging keys and values separately; this one 105.0
+    // (105.4-105.9 in a later same-box A/B, where bin 6 behind bin 5 on the
+    // first side stream measured 105.1-105.6).
     // (Five bins of capacity 1024 << b: one stream 92.0; 4,3 | 2 | 1,0 |
     // long 94.2; 4,1 | 3 | 2,0 | long 96.9; 4,0 | 3 | 2 | 1,long 100.3; the
     // long segments first on their own stream 95-100.)
