@@ -5,7 +5,7 @@ Default workload (the line's `value`): sort_pairs of n = 2^28 uint32 keys
 step = one complete sort of a fresh copy of that input (inputs are resident in
 HBM before the timed region; each step sorts its own copy in place).  The same
 run also measures, each in an object of the line:
-  key_only       the north-star sort: 2^28 u32 keys, with its pass roofline
+  key_only       the key-only sort: 2^28 u32 keys, with its pass roofline
   hbm_ceiling    hand-written read / write / copy kernels, and a scatter copy with
                  one radix pass's write pattern (the practical ceilings)
   configs        plain 2^20 u32 (mt19937 seed 0) latency, 2^27 u64 low-16 and
@@ -250,7 +250,7 @@ def bench_config(rs, name, steps, warmup, descending=False, flags=0):
 
 
 def bench_plain20(rs, steps, warmup):
-    """2^20 u32 keys (raw mt19937 seed 0), microseconds per sort,
+    """Plain 2^20 u32 keys (raw mt19937 seed 0): microseconds per sort,
     direct calls and graph replays (RSORT_FLAG_GRAPH), each call timed alone
     with events around it (the reference's method) and back to back."""
     import torch
@@ -513,7 +513,7 @@ def lognormal_sizes(count, seed=5, median=2048.0, sigma=0.7356, lo=32, hi=1 << 2
 
 
 def bench_segments(rs, a, steps, warmup):
-    """configs[4] batch: one segmented sort of 100k lognormal segments replayed
+    """Segmented batch: one segmented sort of 100k lognormal segments replayed
     from its graph (device time per replay, device checks), and from host C++
     (cli/seg_percall) the same batch's graph replay against one rsort_sort
     call per segment over the first per_call_sample segments."""
@@ -606,7 +606,7 @@ def _submit_raw(rs, srv, kptr, vptr, n):
 
 
 def bench_serve(rs, a):
-    """configs[4] stream: Poisson arrivals through the serving loop, latency
+    """Served stream: Poisson arrivals through the serving loop, latency
     percentiles and throughput; then the same requests submitted at once
     (saturation)."""
     import numpy as np
