@@ -459,8 +459,8 @@ struct Server {
         // enough long requests for one segmented sort: the copy and
         // segmented-sort kernels (each copy lands where it came from, the
         // requests lying back to back in the batch buffer)
-        const uint64_t nlong = kLongEach + 1u;
-        if (e == hipSuccess && nlong * n <= cfg.batch_keys) {
+        const uint64_t nlong = kLongEach + 1u;  // a batch can hold that many only if batch_requests allows
+        if (e == hipSuccess && nlong <= cfg.batch_requests && nlong * n <= cfg.batch_keys) {
             sl.items.clear();
             for (uint64_t i = 0; i < nlong; ++i) {
                 SegReq r{};

@@ -169,3 +169,23 @@ def test_many_long_requests_in_one_batch(rs, orc, kind, kb, vb, desc):
         assert np.array_equal(got_k, ek), f"request {i} (n={m}) keys"
         if vb:
             assert np.array_equal((dv + [dlv])[i].cpu().numpy(), ev), f"request {i} (n={m}) values"
+
+
+def test_small_slots_with_long_requests(rs, orc):
+    """A server whose slots hold fewer requests than its long-request batch
+    size (4): created without error, long requests sorted one by one."""
+    print("[case] serve small slots", flush=True)
+    rng = np.random.default_rng(77)
+    sizes = [20000, 30000, 100, 17000, 5000, 40000, 3]
+    host = [orc.mt19937(700 + i, m) for i, m in enumerate(sizes)]
+    dk = [torch.from_numpy(k.copy()).cuda() for k in host]
+    dv = [torch.from_numpy(rng.integers(0, 2**31, size=m, dtype=np.int64).astype(np.int32)).cuda() for m in sizes]
+    hv = [v.cpu().numpy() for v in dv]
+    with rs.Server(rs.KeyType(0, 4), value_bytes=4, batch_keys=1 << 17, batch_requests=4, depth=2,
+                   mode="direct") as srv:
+        ids = srv.submit_many([k.data_ptr() for k in dk], [v.data_ptr() for v in dv], sizes)
+        for i in ids:
+            srv.wait(i)
+    for k, v, d_k, d_v in zip(host, hv, dk, dv):
+        ek, ev = orc.sort(k, v, 0, 4, False, vb=4)
+        assert np.array_equal(d_k.cpu().numpy(), ek) and np.array_equal(d_v.cpu().numpy(), ev)
