@@ -96,6 +96,17 @@ bool hist_clear_allowed() {
     return v;
 }
 
+// RSORT_AMD_HIST_CLAIM=late (a test setting): the histogram's workgroup 0
+// does not claim the clearing when it starts, so the claim is always made
+// where a workgroup first needs the counts.
+bool hist_late_claim() {
+    static const bool v = [] {
+        const char* e = getenv("RSORT_AMD_HIST_CLAIM");
+        return e && strcmp(e, "late") == 0;
+    }();
+    return v;
+}
+
 // A fresh nonzero 64-bit id per sort call: odd numbers counted up from a
 // random start (random device, clock and process id), so ids never repeat in
 // a process and one left in a workspace by another process matches by chance
@@ -621,6 +632,7 @@ rsort_status enqueue_sort(const Plan& p, void* keys, void* values, const SortBuf
         if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) j.clear_id = next_clear_id();
         else (void)hipGetLastError();
     }
+    j.late_claim = hist_late_claim();
     j.tw = make_tw_masks(p.kind, p.kb, p.desc);
     j.keys = keys;
     j.tmp_keys = b.tmp_keys;

@@ -382,6 +382,7 @@ struct HistParams {
     uint64_t self_clear_units;  // tickets, lookback) itself, in 16-byte units
     uint64_t head_units;   // its first part, header and counts (16-byte units)
     uint64_t clear_id;     // several workgroups clearing: this call's id (nonzero), see k_histogram
+    uint32_t early_claim;  // workgroup 0 claims the clearing when it starts (0: only where counts are needed)
     uint32_t* xcd_lock;    // the device's per-XCD ticket lock word (null: never taken)
     uint32_t want_xcd;     // the passes can run on per-XCD tickets (see k_onesweep)
     TwMasks tw;
@@ -582,6 +583,35 @@ __device__ __forceinline__ void hist_count_one(uint32_t* H, const typename KeyTr
     }
 }
 
+// The histogram's clearing protocol without a fill kernel (k_histogram):
+// thread 0 claims the clearing of the control header and the counts for call
+// `id` if the word holds a stale value (neither id nor id - 1), by a
+// compare-and-swap to id - 1.  `seen` is a value read from the word.
+__device__ __forceinline__ bool hist_claim_clear(uint64_t* word, uint64_t id, uint64_t seen) {
+    while (seen != id && seen != id - 1u) {
+        if (__hip_atomic_compare_exchange_strong(word, &seen, id - 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return true;
+        // `seen` now holds the word's value: id - 1 or id (only this call's
+        // workgroups write it while the call runs)
+    }
+    return false;
+}
+
+// Every thread of the claiming workgroup: coherent zeros over the header (but
+// its clear_id word) and the counts, then, once they have completed, thread 0
+// publishes the call's id.
+__device__ __forceinline__ void hist_clear_head(const HistParams& a, uint32_t tid) {
+    uint32_t* w32 = (uint32_t*)a.self_clear;
+    const uint32_t words = (uint32_t)(a.head_units * 4u);
+    constexpr uint32_t cw = (uint32_t)(offsetof(Ctrl, clear_id) / 4u);
+    for (uint32_t i = tid; i < words; i += kHistThreads)
+        if (i != cw && i != cw + 1u) __hip_atomic_store(w32 + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&a.ctrl->clear_id, a.clear_id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int KB, bool WIDE>
 __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     using Reg = typename KeyTraits<KB>::Reg;
@@ -612,6 +642,7 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     __shared__ Off s_wsum[4];
     __shared__ uint32_t s_last;
     __shared__ uint32_t s_skip;
+    __shared__ uint32_t s_mine;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t w = tid >> 6;
@@ -631,23 +662,29 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
         for (uint64_t u = hdr + tid; u < a.self_clear_units; u += kHistThreads)
             a.self_clear[u] = make_uint4(0u, 0u, 0u, 0u);
     } else if (a.self_clear_units) {
-        // Several workgroups and no fill kernel.  Workgroup 0 clears the
-        // header and the counts with coherent (write-through) stores, waits
-        // for them to complete, then stores this call's id into the header;
-        // every workgroup waits to read that id before its first access to
-        // the header or the counts (below), so the id a stale header may hold
-        // is never this call's (ids are unique per process).  Workgroup 0 is
-        // dispatched first and waits on nothing, so the others' wait ends.
+        // Several workgroups and no fill kernel.  The header word clear_id
+        // holds this call's id (odd, unique per process) once the header and
+        // the counts are cleared for this call, id - 1 while a workgroup clears
+        // them, and anything else (a stale value) before.  One running
+        // workgroup claims the clearing by a compare-and-swap from a stale
+        // value to id - 1: workgroup 0 at its start, or, if it has not got
+        // there yet, whichever workgroup first reaches the point where it
+        // needs the counts (below).  The claimer clears the header (but
+        // clear_id) and the counts with coherent stores, waits for them, then
+        // stores id.  Every workgroup waits to read id before its first access
+        // to the header or the counts; it only ever waits on a claimer that
+        // was running when it claimed and waits on nothing, so the wait ends
+        // whatever the dispatch order and whatever else runs on the device.
         // The ticket and lookback words, read only by the pass kernels, are
         // cleared by every workgroup with plain stores.
-        if (blockIdx.x == 0) {
-            uint32_t* w32 = (uint32_t*)a.self_clear;
-            const uint32_t words = (uint32_t)(a.head_units * 4u);
-            for (uint32_t i = tid; i < words; i += kHistThreads)
-                __hip_atomic_store(w32 + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __builtin_amdgcn_s_waitcnt(0);
+        if (blockIdx.x == 0 && a.early_claim) {
+            if (tid == 0) s_mine = hist_claim_clear(&a.ctrl->clear_id, a.clear_id,
+                                                    __hip_atomic_load(&a.ctrl->clear_id, __ATOMIC_RELAXED,
+                                                                      __HIP_MEMORY_SCOPE_AGENT))
+                                       ? 1u
+                                       : 0u;
             __syncthreads();
-            if (tid == 0) __hip_atomic_store(&a.ctrl->clear_id, a.clear_id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (s_mine) hist_clear_head(a, tid);
         }
         for (uint64_t u = a.head_units + (uint64_t)blockIdx.x * kHistThreads + tid; u < a.self_clear_units;
              u += (uint64_t)gridDim.x * kHistThreads)
@@ -744,11 +781,24 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(HistParams a) {
     __syncthreads();
     RSA_MARK(16);
     if (a.clear_id) {
-        // the header and the counts are cleared for this call (see above)
-        if (tid == 0)
-            while (__hip_atomic_load(&a.ctrl->clear_id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.clear_id)
+        // the header and the counts must be cleared for this call (see above):
+        // read the id, or claim the clearing if nobody has, or wait for the
+        // running claimer
+        if (tid == 0) {
+            uint32_t mine = 0u;
+            while (true) {
+                const uint64_t cw = __hip_atomic_load(&a.ctrl->clear_id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (cw == a.clear_id) break;
+                if (cw != a.clear_id - 1u && hist_claim_clear(&a.ctrl->clear_id, a.clear_id, cw)) {
+                    mine = 1u;
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(1);
+            }
+            s_mine = mine;
+        }
         __syncthreads();
+        if (s_mine) hist_clear_head(a, tid);
     }
 
     Off* counts = (Off*)a.counts;

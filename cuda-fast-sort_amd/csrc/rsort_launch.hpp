@@ -28,6 +28,7 @@ struct SortJob {
     uint32_t* xcd_lock;   // the device's per-XCD ticket lock word (device address; null: global ticket only)
     uint32_t hist_direct_max;  // histogram grids up to this fold by direct device atomics
     uint64_t clear_id;    // nonzero: the histogram may clear the control region itself (unique per call)
+    bool late_claim;      // test setting: no workgroup claims that clearing before it needs the counts
     TwMasks tw;
     void* keys;           // caller buffer (in/out)
     void* vals;           // caller buffer (in/out) or index buffer

@@ -108,6 +108,7 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     h.self_clear_units = self_clear ? clear_units : 0u;
     h.head_units = head_units;
     h.clear_id = self_clear && hgrid > 1u ? j.clear_id : 0u;
+    h.early_claim = j.late_claim ? 0u : 1u;
     const PassKernel pk = choose_pass_kernel<KB, VB, WIDE>(j);
     h.xcd_lock = pk.xcd_ok ? j.xcd_lock : nullptr;
     h.want_xcd = pk.xcd_ok ? 1u : 0u;

@@ -14,6 +14,16 @@
 //           if it takes the device's lock when it starts (else the global
 //           ticket) and frees it when its last tile finishes: never deadlocks;
 //   global  every kernel on the global ticket: never deadlocks.
+// Two more put a histogram kernel that clears the control header itself
+// (k_histogram without a fill kernel) beside a pass kernel holding the lock:
+//   hist-wait   every histogram workgroup waits until workgroup 0 has cleared
+//               (workgroup 0 goes first only on its own XCD): the adversarial
+//               schedule (XCD 0 favours the pass kernel) deadlocks;
+//   hist-claim  the library's protocol: workgroup 0 claims the clearing when
+//               it starts, and a workgroup that needs the counts before anyone
+//               has claimed claims it itself; a claimer is running and waits on
+//               nothing, so no histogram workgroup waits for long: never
+//               deadlocks.
 // Prints the deadlock counts; exit status 0 when they are as stated.
 #include <stdint.h>
 #include <stdio.h>
@@ -31,6 +41,8 @@ constexpr uint32_t kChunk = 16;
 
 struct Kernel {
     uint32_t tiles = 0, grid = 0;
+    bool hist = false;       // a histogram kernel (no tickets): workgroup b is "tile" b
+    bool cleared = false;    // its control header is cleared for this call
     uint64_t arrive = 0;  // step at which it is launched
     bool started = false, xcd = false, holds_lock = false;
     uint32_t counter[kParts] = {0};
@@ -47,7 +59,7 @@ struct Slot {
     uint32_t tile = 0;
 };
 
-enum Mode { kAllXcd, kLock, kAllGlobal };
+enum Mode { kAllXcd, kLock, kAllGlobal, kHistWait, kHistClaim };
 
 // Returns true when the schedule deadlocks.
 bool simulate(Mode mode, uint32_t slots_per_xcd, std::vector<Kernel> ks, bool adversarial, uint32_t seed) {
@@ -63,8 +75,9 @@ bool simulate(Mode mode, uint32_t slots_per_xcd, std::vector<Kernel> ks, bool ad
         for (auto& k : ks) {
             if (k.started || step < k.arrive) continue;
             k.started = true;
+            if (k.hist) continue;
             if (mode == kAllXcd) k.xcd = true;
-            else if (mode == kLock && !lock) k.xcd = k.holds_lock = lock = true;
+            else if (mode != kAllGlobal && !lock) k.xcd = k.holds_lock = lock = true;
         }
     };
     auto take_ticket = [&](Kernel& k, uint32_t b) -> uint32_t {
@@ -93,8 +106,14 @@ bool simulate(Mode mode, uint32_t slots_per_xcd, std::vector<Kernel> ks, bool ad
         // candidate events: finish a workgroup whose lower tiles are all handed out,
         // or dispatch on an XCD with a free slot
         std::vector<uint32_t> runnable, dispatchable;
-        for (uint32_t i = 0; i < slots.size(); ++i)
-            if (slots[i].k >= 0 && slots[i].tile < ks[slots[i].k].lowest_unclaimed) runnable.push_back(i);
+        for (uint32_t i = 0; i < slots.size(); ++i) {
+            if (slots[i].k < 0) continue;
+            const Kernel& k = ks[slots[i].k];
+            // a histogram workgroup needs the cleared header; under the claim
+            // protocol it clears it itself if nobody has claimed the clearing
+            const bool ready = k.hist ? (k.cleared || mode == kHistClaim) : slots[i].tile < k.lowest_unclaimed;
+            if (ready) runnable.push_back(i);
+        }
         for (uint32_t x = 0; x < kXcds; ++x) {
             bool free = false, work = false;
             for (uint32_t s = 0; s < slots_per_xcd; ++s) free = free || slots[x * slots_per_xcd + s].k < 0;
@@ -112,7 +131,13 @@ bool simulate(Mode mode, uint32_t slots_per_xcd, std::vector<Kernel> ks, bool ad
             !dispatchable.empty() && (runnable.empty() || adversarial || (rng() & 1u));
         if (!do_dispatch) {
             const uint32_t i = runnable[rng() % runnable.size()];
-            finish_tile(ks[slots[i].k], slots[i].tile);
+            Kernel& k = ks[slots[i].k];
+            if (k.hist) {
+                k.cleared = true;  // it read the id, or claimed and cleared
+                ++k.finished;
+            } else {
+                finish_tile(k, slots[i].tile);
+            }
             slots[i].k = -1;
             continue;
         }
@@ -130,6 +155,18 @@ bool simulate(Mode mode, uint32_t slots_per_xcd, std::vector<Kernel> ks, bool ad
         Kernel& k = ks[ki];
         const uint32_t b = k.next_wg[x];
         k.next_wg[x] += kXcds;
+        if (k.hist) {
+            if (b == 0) k.cleared = true;  // workgroup 0 clears when it starts
+            for (uint32_t s = 0; s < slots_per_xcd; ++s) {
+                Slot& sl = slots[x * slots_per_xcd + s];
+                if (sl.k < 0) {
+                    sl.k = ki;
+                    sl.tile = b;
+                    break;
+                }
+            }
+            continue;
+        }
         const uint32_t t = take_ticket(k, b);
         if (t >= k.tiles) continue;  // past the last tile: ends at once
         for (uint32_t s = 0; s < slots_per_xcd; ++s) {
@@ -156,24 +193,36 @@ std::vector<Kernel> make_kernels(std::mt19937& rng, uint32_t count, bool same_st
 
 }  // namespace
 
+// A pass kernel and a histogram kernel of 16-256 workgroups that clears its
+// own header (its grid is at most one workgroup per CU: 32 per XCD).
+std::vector<Kernel> make_pass_and_hist(std::mt19937& rng, bool same_start) {
+    std::vector<Kernel> ks = make_kernels(rng, 2u, same_start);
+    ks[1].hist = true;
+    ks[1].grid = ks[1].tiles = 16u + rng() % 241u;
+    return ks;
+}
+
 int main() {
     int errors = 0;
-    const char* names[] = {"xcd", "lock", "global"};
-    for (Mode m : {kAllXcd, kLock, kAllGlobal}) {
+    const char* names[] = {"xcd", "lock", "global", "hist-wait", "hist-claim"};
+    for (Mode m : {kAllXcd, kLock, kAllGlobal, kHistWait, kHistClaim}) {
         uint32_t dead_adv = 0, dead_rand = 0, runs_adv = 0, runs_rand = 0;
         for (uint32_t seed = 1; seed <= 400; ++seed) {
             std::mt19937 rng(seed);
             const uint32_t spx = 4u + rng() % 13u;  // 4..16 slots per XCD
             const bool adversarial = seed % 2 == 0;
-            auto ks = make_kernels(rng, adversarial ? 2u : 2u + rng() % 3u, adversarial);
+            const bool hist = m == kHistWait || m == kHistClaim;
+            auto ks = hist ? make_pass_and_hist(rng, adversarial)
+                           : make_kernels(rng, adversarial ? 2u : 2u + rng() % 3u, adversarial);
             const bool dead = simulate(m, spx, ks, adversarial, seed * 977u);
             (adversarial ? dead_adv : dead_rand) += dead;
             (adversarial ? runs_adv : runs_rand) += 1;
         }
         printf("%-6s deadlocks: adversarial %u/%u, random %u/%u\n", names[m], dead_adv, runs_adv, dead_rand,
                runs_rand);
-        if (m == kAllXcd && dead_adv == 0) ++errors;        // the hazard must show
-        if (m != kAllXcd && dead_adv + dead_rand != 0) ++errors;  // the rule must remove it
+        const bool hazard = m == kAllXcd || m == kHistWait;
+        if (hazard && dead_adv == 0) ++errors;                 // the hazard must show
+        if (!hazard && dead_adv + dead_rand != 0) ++errors;    // the rule must remove it
     }
     printf("dispatch simulation: %d errors\n", errors);
     return errors == 0 ? 0 : 1;

@@ -5,7 +5,8 @@ two-level histogram fold at every grid size (RSORT_AMD_HIST_DIRECT=0), the
 single global tile ticket (RSORT_AMD_XCD_TICKETS=0), the onesweep path
 for sorts small enough for one LDS bin (RSORT_AMD_SMALL_LDS=0) and the fill
 kernel clearing the control region before every histogram
-(RSORT_AMD_HIST_CLEAR=0).
+(RSORT_AMD_HIST_CLEAR=0), and the histogram's clearing claimed only where a
+workgroup first needs the counts (RSORT_AMD_HIST_CLAIM=late).
 
 The library reads them once per process, so test_gpu_env_paths.py runs this
 file in a process of its own per setting.  Cases: u32 keys and u32/u32 pairs

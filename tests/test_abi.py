@@ -120,7 +120,10 @@ def test_lookback_protocol_simulation(tmp_path):
 def test_dispatch_model_per_xcd_lock(tmp_path):
     """Two to four pass kernels sharing the device's XCDs: per-XCD tickets on
     more than one of them can deadlock; with the device lock (one per-XCD
-    kernel at a time, the others on the global ticket) no schedule does."""
+    kernel at a time, the others on the global ticket) no schedule does.  A
+    histogram that clears its own header beside a lock-holding pass kernel:
+    waiting for workgroup 0 to clear can deadlock; the claim protocol the
+    kernel uses (a running claimer that waits on nothing) never does."""
     exe = tmp_path / "dispatch_sim"
     src = os.path.join(ROOT, "tests", "dispatch_sim.cpp")
     subprocess.run(["g++", "-O2", "-std=c++17", src, "-o", str(exe)], check=True)
@@ -128,6 +131,7 @@ def test_dispatch_model_per_xcd_lock(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "lock   deadlocks: adversarial 0/" in r.stdout and ": 0 errors" in r.stdout
+    assert "hist-claim deadlocks: adversarial 0/200, random 0/200" in r.stdout
 
 
 # ---------------------------------------------------------------------------

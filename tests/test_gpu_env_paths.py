@@ -18,6 +18,7 @@ SETTINGS = {
     "global_tile_ticket": {"RSORT_AMD_XCD_TICKETS": "0"},
     "small_sorts_on_onesweep": {"RSORT_AMD_SMALL_LDS": "0"},
     "fill_kernel_clears": {"RSORT_AMD_HIST_CLEAR": "0"},
+    "histogram_claims_late": {"RSORT_AMD_HIST_CLAIM": "late"},
 }
 
 
