@@ -1139,6 +1139,12 @@ __global__ __launch_bounds__(THREADS, pass_min_waves(KB, VB, K)) void k_onesweep
     // the two header words every pass reads first, in one load (the second,
     // xcd_mode, picks the ticket counter below)
     const uint2 hdr = *(const uint2*)&a.ctrl->skip_mask;
+    // every kernel argument the prologue and the tile need, loaded together
+    // with that word: otherwise the compiler loads the argument lines one
+    // miss after another, ahead of the ticket
+    asm volatile("" ::"s"(a.keys_a), "s"(a.keys_b), "s"(a.vals_a), "s"(a.vals_b), "s"(a.gp), "s"(a.lookback),
+                 "s"(a.lb_groups), "s"(a.tickets_xcd), "s"(a.xcd_ok), "s"(a.n), "s"(a.num_tiles), "s"(a.tw.fmask),
+                 "s"(a.tw.active));
     const PassRun pr = pass_run(hdr.x, a.passes, a.pass);
     if (!pr.runs) return;  // uniform digit: pass is the identity
     PassIo<KB, VB> io;
