@@ -790,20 +790,39 @@ rsort_status rsort_serve_submit_many(rsort_server* srv, void* const* keys, void*
     return RSORT_SUCCESS;
 }
 
-rsort_status rsort_serve_wait(rsort_server* srv, uint64_t request_id, uint64_t timeout_us, rsort_request_info* info) {
-    Server* s = (Server*)srv;
-    if (!s) return RSORT_ERROR_INVALID_ARGUMENT;
-    std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->req.find(request_id);
-    if (it == s->req.end()) return RSORT_ERROR_INVALID_ARGUMENT;
-    const auto deadline = rsa::Clock::now() + std::chrono::microseconds(timeout_us);
-    while (!it->second.done && s->error == RSORT_SUCCESS) {
-        if (timeout_us == 0 || s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) {
-            if (!it->second.done) return RSORT_ERROR_BUSY;
+namespace {
+
+// Waits, under the server's lock, until request `id` is done, the server has
+// failed, or the deadline passes (timeout_us: 0 = poll, RSORT_SERVE_WAIT_FOREVER
+// or more = no limit).  SUCCESS when done; BUSY on time-out; INVALID_ARGUMENT
+// when the record is unknown (never submitted, or released by another wait).
+rsort_status wait_done(Server* s, std::unique_lock<std::mutex>& lk, uint64_t id, uint64_t timeout_us,
+                       rsa::Clock::time_point deadline) {
+    while (true) {
+        auto it = s->req.find(id);
+        if (it == s->req.end()) return RSORT_ERROR_INVALID_ARGUMENT;
+        if (it->second.done) return RSORT_SUCCESS;
+        if (s->error != RSORT_SUCCESS) return s->error;
+        if (timeout_us == 0) return RSORT_ERROR_BUSY;
+        if (timeout_us >= RSORT_SERVE_WAIT_FOREVER) {
+            s->cv_done.wait(lk);
+        } else if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) {
+            it = s->req.find(id);
+            if (it == s->req.end()) return RSORT_ERROR_INVALID_ARGUMENT;
+            return it->second.done ? RSORT_SUCCESS : RSORT_ERROR_BUSY;
         }
-        it = s->req.find(request_id);
     }
-    if (!it->second.done) return s->error;
+}
+
+rsa::Clock::time_point deadline_after(uint64_t timeout_us) {
+    const uint64_t us = timeout_us < RSORT_SERVE_WAIT_FOREVER ? timeout_us : 0u;
+    return rsa::Clock::now() + std::chrono::microseconds((int64_t)us);
+}
+
+// Fills `info` (if any) from a done record and releases it.
+void release(Server* s, uint64_t id, rsort_request_info* info) {
+    auto it = s->req.find(id);
+    if (it == s->req.end()) return;
     if (info) {
         info->latency_us = rsa::us_since(it->second.t_submit, it->second.t_done);
         info->queue_us = rsa::us_since(it->second.t_submit, it->second.t_launch);
@@ -811,6 +830,32 @@ rsort_status rsort_serve_wait(rsort_server* srv, uint64_t request_id, uint64_t t
         info->n = it->second.n;
     }
     s->req.erase(it);
+}
+
+}  // namespace
+
+rsort_status rsort_serve_wait(rsort_server* srv, uint64_t request_id, uint64_t timeout_us, rsort_request_info* info) {
+    Server* s = (Server*)srv;
+    if (!s) return RSORT_ERROR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lk(s->mu);
+    const rsort_status st = wait_done(s, lk, request_id, timeout_us, deadline_after(timeout_us));
+    if (st == RSORT_SUCCESS) release(s, request_id, info);
+    return st;
+}
+
+rsort_status rsort_serve_wait_many(rsort_server* srv, const uint64_t* request_ids, uint64_t count,
+                                   uint64_t timeout_us, rsort_request_info* infos) {
+    Server* s = (Server*)srv;
+    if (!s || (count && !request_ids)) return RSORT_ERROR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (uint64_t i = 0; i < count; ++i)
+        if (s->req.find(request_ids[i]) == s->req.end()) return RSORT_ERROR_INVALID_ARGUMENT;
+    const auto deadline = deadline_after(timeout_us);
+    for (uint64_t i = 0; i < count; ++i) {
+        const rsort_status st = wait_done(s, lk, request_ids[i], timeout_us, deadline);
+        if (st != RSORT_SUCCESS) return st;  // nothing released
+    }
+    for (uint64_t i = 0; i < count; ++i) release(s, request_ids[i], infos ? infos + i : nullptr);
     return RSORT_SUCCESS;
 }
 

@@ -150,6 +150,7 @@ def lib() -> ctypes.CDLL:
         L.rsort_serve_create.argtypes = [ctypes.POINTER(ServeConfig), ctypes.POINTER(vp)]
         L.rsort_serve_submit.argtypes = [vp, vp, vp, u64, ctypes.POINTER(ctypes.c_uint64)]
         L.rsort_serve_wait.argtypes = [vp, u64, u64, ctypes.POINTER(RequestInfo)]
+        L.rsort_serve_wait_many.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), u64, u64, ctypes.POINTER(RequestInfo)]
         L.rsort_serve_submit_many.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                               ctypes.POINTER(ctypes.c_uint64), u64, ctypes.POINTER(ctypes.c_uint64)]
         L.rsort_serve_flush.argtypes = [vp]
@@ -442,18 +443,37 @@ class Server:
 
     def submit_many(self, key_ptrs, val_ptrs, sizes) -> list[int]:
         """Submits requests given as device addresses and sizes in one call."""
+        import numpy as np
+
         count = len(sizes)
-        kp = (ctypes.c_void_p * count)(*key_ptrs)
-        vp_ = (ctypes.c_void_p * count)(*(val_ptrs if val_ptrs is not None else [None] * count))
-        nn = (ctypes.c_uint64 * count)(*sizes)
-        ids = (ctypes.c_uint64 * count)()
-        _check(lib().rsort_serve_submit_many(self._h, kp, vp_ if val_ptrs is not None else None, nn, count, ids))
-        return list(ids)
+        pv = ctypes.POINTER(ctypes.c_void_p)
+        pu = ctypes.POINTER(ctypes.c_uint64)
+        kp = np.ascontiguousarray(key_ptrs, dtype=np.uint64)
+        vp_ = np.ascontiguousarray(val_ptrs, dtype=np.uint64) if val_ptrs is not None else None
+        nn = np.ascontiguousarray(sizes, dtype=np.uint64)
+        if kp.size != count or nn.size != count or (vp_ is not None and vp_.size != count):
+            raise ValueError("key_ptrs, val_ptrs and sizes must have one entry per request")
+        ids = np.zeros(count, dtype=np.uint64)
+        _check(lib().rsort_serve_submit_many(self._h, kp.ctypes.data_as(pv),
+                                             vp_.ctypes.data_as(pv) if vp_ is not None else None,
+                                             nn.ctypes.data_as(pu), count, ids.ctypes.data_as(pu)))
+        return ids.tolist()
 
     def wait(self, request_id: int, timeout_us: int = 10_000_000) -> RequestInfo:
         info = RequestInfo()
         _check(lib().rsort_serve_wait(self._h, request_id, timeout_us, ctypes.byref(info)))
         return info
+
+    def wait_many(self, request_ids, timeout_us: int = 10_000_000):
+        """Waits for all the requests in one call and releases them; returns
+        their RequestInfo records (a ctypes array, in the order given)."""
+        import numpy as np
+
+        ids = np.ascontiguousarray(request_ids, dtype=np.uint64)
+        infos = (RequestInfo * ids.size)()
+        _check(lib().rsort_serve_wait_many(self._h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ids.size,
+                                           timeout_us, infos))
+        return infos
 
     def flush(self) -> None:
         _check(lib().rsort_serve_flush(self._h))

@@ -293,7 +293,8 @@ int rsort_tuning_load(const char* path);
  * waited window_us (0: as soon as a slot is free), or on flush.  Direct form
  * (default, power-of-two value widths): requests are binned by size on the
  * host and each LDS bin with requests is one launch sorting every request in
- * place; longer requests get their own rsort_sort.  Batched form
+ * place; up to four longer requests get their own rsort_sort, more are sorted
+ * together by one segmented sort in the slot's buffers.  Batched form
  * (RSORT_SERVE_BATCHED): gather -> segmented sort -> scatter through the
  * slot's buffers, captured once into a hipGraph unless RSORT_SERVE_NO_GRAPH.
  * Admission: a request must fit one slot (UNSUPPORTED otherwise) and the
@@ -336,8 +337,17 @@ rsort_status rsort_serve_submit(rsort_server* s, void* keys, void* values, uint6
 /* Submits `count` requests at once (all or none; ids in request_ids). */
 rsort_status rsort_serve_submit_many(rsort_server* s, void* const* keys, void* const* values, const uint64_t* n,
                                      uint64_t count, uint64_t* request_ids);
-/* Blocks until the request is done (BUSY after timeout_us; 0 = poll) and releases its record. */
+/* Blocks until the request is done (BUSY after timeout_us; 0 = poll;
+ * RSORT_SERVE_WAIT_FOREVER or more = no limit) and releases its record. */
+#define RSORT_SERVE_WAIT_FOREVER (1ull << 62)
 rsort_status rsort_serve_wait(rsort_server* s, uint64_t request_id, uint64_t timeout_us, rsort_request_info* info);
+/* Waits for `count` distinct requests under one time limit (as
+ * rsort_serve_wait) and then releases them all, filling infos[i] when infos is
+ * not NULL; on BUSY or an error none is released.  No reference counterpart:
+ * the batch counterpart of rsort_serve_wait, as rsort_serve_submit_many is of
+ * rsort_serve_submit. */
+rsort_status rsort_serve_wait_many(rsort_server* s, const uint64_t* request_ids, uint64_t count, uint64_t timeout_us,
+                                   rsort_request_info* infos);
 /* Closes the open batch now and waits until nothing is queued or in flight. */
 rsort_status rsort_serve_flush(rsort_server* s);
 rsort_status rsort_serve_stats(rsort_server* s, rsort_serve_stats_info* out);

@@ -189,3 +189,27 @@ def test_small_slots_with_long_requests(rs, orc):
     for k, v, d_k, d_v in zip(host, hv, dk, dv):
         ek, ev = orc.sort(k, v, 0, 4, False, vb=4)
         assert np.array_equal(d_k.cpu().numpy(), ek) and np.array_equal(d_v.cpu().numpy(), ev)
+
+
+def test_wait_many(rs, orc):
+    """wait_many: BUSY while the batch is held open (nothing released), an
+    unknown id refused, then every request's info at once after a flush."""
+    print("[case] serve wait_many", flush=True)
+    host = [orc.mt19937(900 + i, m) for i, m in enumerate([500, 0, 3000, 20000, 7])]
+    dk = [torch.from_numpy(k.copy()).cuda() for k in host]
+    with rs.Server(rs.KeyType(0, 4), batch_keys=1 << 20, batch_requests=64, window_us=5_000_000, depth=2) as srv:
+        ids = [srv.submit(k) for k in dk]
+        with pytest.raises(rs.RsortError) as e:
+            srv.wait_many(ids, timeout_us=0)  # the window keeps the batch open
+        assert e.value.status == 5
+        with pytest.raises(rs.RsortError) as e:
+            srv.wait_many(ids + [10**9])
+        assert e.value.status == 1
+        srv.flush()
+        infos = srv.wait_many(ids)
+        assert [x.n for x in infos] == [k.size for k in host]
+        assert len({x.batch for x in infos}) == 1 and all(x.latency_us >= x.queue_us for x in infos)
+        with pytest.raises(rs.RsortError):
+            srv.wait(ids[0], timeout_us=0)  # released
+    for k, d in zip(host, dk):
+        assert np.array_equal(d.cpu().numpy(), np.sort(k))
