@@ -62,3 +62,11 @@ def test_overlapping_sorts_mixed_widths(rs, orc):
 def test_sort_alone_after_overlap(rs, orc):
     n = (1 << 22) + 77
     _sorts_on_streams(rs, orc, [(0, 4, 4, False, n, 31)], rounds=2)
+
+
+def test_big_sort_beside_small_sorts(rs, orc):
+    """A large sort (its passes on the per-XCD tickets when it takes the lock)
+    launched first, then six small sorts whose histograms clear their control
+    headers themselves, each on its own stream: every result bit-exact."""
+    cases = [(0, 4, 4, False, 1 << 25, 41)] + [(0, 4, 0, False, (1 << 20) + 13 * i, 42 + i) for i in range(6)]
+    _sorts_on_streams(rs, orc, cases, rounds=2)
