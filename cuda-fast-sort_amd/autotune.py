@@ -38,6 +38,8 @@ sys.path.insert(0, HERE)
 TUNED = [(4, 0), (4, 4), (8, 0), (8, 8), (4, 8), (8, 4), (16, 0), (16, 8)]
 # flat sorts are measured at 2^log2n keys, fewer where one array would pass 2 GiB
 ARRAY_BYTES_MAX = 1 << 31
+# untimed u32/u32 sorts before the first measurement
+WARMUP_SORTS = 12
 # pass-kernel tile candidates (kTileCands in csrc/rsort_tuning.hpp)
 TILE_CANDIDATES = 4
 
@@ -178,6 +180,11 @@ def main():
 
     assert torch.cuda.is_available(), "the autotuner measures on a GPU"
     rs.tuning_load("")  # start from the built-in defaults
+    # The device runs its first sorts of a process slower (up to ~10 % for
+    # the first eight or so); warm it up before anything is timed, or the
+    # first candidates measured carry that transient.
+    for _ in range(WARMUP_SORTS // 2):
+        time_flat(rs, torch, 4, 4, 1 << min(a.log2n, 28), 1)  # two sorts each
     sizes = lognormal_sizes(a.segments)
     lines, report = [], []
     widths = TUNED if not a.widths else [tuple(int(x) for x in w.split(":")) for w in a.widths.split(",")]
