@@ -236,7 +236,7 @@ def main():
     header = [f"# rsort tuning table for {getattr(props, 'gcnArchName', 'gfx950')} ({props.name}), written by autotune.py",
               f"# {time.strftime('%Y-%m-%d')}: flat sorts of 2^{a.log2n} uniform keys (fewer where an array "
               f"would pass {ARRAY_BYTES_MAX >> 30} GiB), segmented sorts of "
-              f"{a.segments} lognormal segments, mean of {a.reps} runs each",
+              f"{a.segments} lognormal segments, mean of {a.reps} runs each, after {WARMUP_SORTS} untimed sorts",
               "# key_bytes value_bytes tile_candidate hist_blocks_per_cu lds_threshold small_tile_candidate small_log2n "
               "tiny_tile_candidate tiny_log2n"]
     text = "\n".join(header + lines) + "\n"
