@@ -74,14 +74,15 @@ hipError_t launch_core_t(const SortJob& j, hipStream_t s) {
     // 1. control header, global digit counts, tickets and lookback words
     //    (flag 0: never ready in any epoch), contiguous in the workspace: one
     //    fill, or none when the histogram clears them itself (a single
-    //    workgroup, or a grid within one workgroup per CU given a call id;
-    //    the fill stays for large grids and for captured launches, whose
-    //    replays would repeat one id)
+    //    workgroup, or, given a call id, a region of at most
+    //    kSelfClearMaxBytes; the fill stays for larger regions, where its
+    //    cost is lost in the sort, and for captured launches, whose replays
+    //    would repeat one id)
     const uint64_t head_units = (uint64_t)((sizeof(Ctrl) + (size_t)P * kRadix * sizeof(Off)) / 16u);
     const uint64_t clear_units = head_units + (uint64_t)(j.lb_bytes / 16u);
     const uint32_t hg = grid_for(j.n, hist_keys_per_block_iter(KB), j.num_cus, j.hist_bpc ? j.hist_bpc : 1u);
     const uint32_t hgrid = hg < kHistMaxGrid ? hg : kHistMaxGrid;
-    const bool self_clear = hgrid == 1u || (j.clear_id != 0u && hgrid <= (uint32_t)j.num_cus);
+    const bool self_clear = hgrid == 1u || (j.clear_id != 0u && clear_units * 16u <= kSelfClearMaxBytes);
     if (!self_clear)
         hipLaunchKernelGGL(k_fill_zero16, dim3(grid_for(clear_units, 256u * 8u, j.num_cus, 4)), dim3(256), 0, s,
                            (uint4*)j.ctrl, clear_units);

@@ -25,6 +25,12 @@ constexpr int kHistGroup = 32;
 // Upper bound of the histogram grid (32 groups, the arrival counters that fit
 // in the control header).
 constexpr uint32_t kHistMaxGrid = 32 * kHistGroup;
+
+// The histogram kernel clears the control region (header, counts, tickets,
+// lookback words) itself, with no fill kernel, up to this size (a u32 sort
+// of up to 2^25 keys: 1.5 KiB per 16K-key tile).  Above it, the plain stores
+// cost the histogram more than the fill kernel costs the sort.
+constexpr uint64_t kSelfClearMaxBytes = 4ull << 20;
 // Histogram grids up to this size fold by device atomics from every
 // workgroup (fewer coherent round trips; the two-level fold above it keeps
 // the device atomics few).
