@@ -93,6 +93,18 @@ def test_u32_pairs_large(rs, orc, n, desc):
     check(rs, orc, keys, index_values(4, n), 0, 4, 4, desc)
 
 
+@pytest.mark.parametrize("vb", [0, 4])
+def test_tile_boundary_sizes(rs, orc, vb):
+    """u32 sizes one key either side of whole tiles, for the tile of each
+    size tier of the committed tuning (1024 x 8 or 1024 x 10 below 2^22 keys,
+    1024 x 16 from there), and of one histogram workgroup's share."""
+    sizes = {8191, 8193, 10239, 10241, 16383, 16385, 16384 * 3 + 1, 8192 * 128 - 1, 8192 * 128 + 1,
+             10240 * 103 + 1, (1 << 22) - 1, (1 << 22) + 1, (1 << 22) + 16383}
+    for i, n in enumerate(sorted(sizes)):
+        keys = orc.mt19937(60 + i, n).view(np.uint8)
+        check(rs, orc, keys, index_values(4, n) if vb else None, 0, 4, vb, bool(i & 1))
+
+
 def test_cfg_plain_keys_mt19937_seed0(rs, orc):
     """n = 2^20 uint32 keys = raw mt19937(seed 0), key-only ascending."""
     n = 1 << 20
