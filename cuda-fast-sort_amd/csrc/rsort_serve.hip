@@ -286,6 +286,7 @@ struct Server {
     }
 
     static constexpr size_t kForkRequests = 32;
+    static constexpr size_t kMergeRequests = 4;  // batches this small run their bins as one launch
 #ifndef RSA_SERVE_LONG_EACH
 #define RSA_SERVE_LONG_EACH 4
 #endif
@@ -334,6 +335,22 @@ struct Server {
             bin[i] = b;
             if (b < (uint32_t)kSegBins) ++in_bin[b];
             else if (n >= 2) longs.push_back(i);
+        }
+        // A few requests in different bins share the largest of those bins:
+        // one launch instead of one per bin (a bin's workgroup sorts any
+        // request up to its capacity), which shortens a small batch on one
+        // stream by a kernel per extra bin.
+        if (nr <= kMergeRequests) {
+            uint32_t top = 0, used = 0;
+            for (int b = 0; b < kSegBins; ++b)
+                if (in_bin[b]) top = (uint32_t)b, ++used;
+            if (used > 1) {
+                for (size_t i = 0; i < nr; ++i)
+                    if (bin[i] < (uint32_t)kSegBins) bin[i] = top;
+                for (int b = 0; b < kSegBins; ++b) in_bin[b] = 0;
+                for (size_t i = 0; i < nr; ++i)
+                    if (bin[i] < (uint32_t)kSegBins) ++in_bin[bin[i]];
+            }
         }
         uint32_t at = 0;
         for (int b = 0; b < kSegBins; ++b) {

@@ -213,3 +213,23 @@ def test_wait_many(rs, orc):
             srv.wait(ids[0], timeout_us=0)  # released
     for k, d in zip(host, dk):
         assert np.array_equal(d.cpu().numpy(), np.sort(k))
+
+
+def test_small_batch_bins_merged(rs, orc):
+    """A batch of three requests from three LDS bins (run as one launch of
+    the largest of them) and one of two requests: every request exact."""
+    print("[case] serve merged bins", flush=True)
+    for sizes in ([600, 3000, 12000], [5, 9000]):
+        host = [orc.mt19937(1100 + m, m) for m in sizes]
+        vals = [np.arange(m, dtype=np.int32) for m in sizes]
+        dk = [torch.from_numpy(k.copy()).cuda() for k in host]
+        dv = [torch.from_numpy(v.copy()).cuda() for v in vals]
+        with rs.Server(rs.KeyType(0, 4), value_bytes=4, batch_keys=1 << 20, batch_requests=64, window_us=5_000_000,
+                       depth=2) as srv:
+            ids = srv.submit_many([k.data_ptr() for k in dk], [v.data_ptr() for v in dv], sizes)
+            srv.flush()
+            infos = srv.wait_many(ids)
+            assert len({x.batch for x in infos}) == 1
+        for k, v, d_k, d_v in zip(host, vals, dk, dv):
+            ek, ev = orc.sort(k, v, 0, 4, False, vb=4)
+            assert np.array_equal(d_k.cpu().numpy(), ek) and np.array_equal(d_v.cpu().numpy(), ev)
